@@ -1,0 +1,70 @@
+// Solver options -- the reference's pdlp::PdlpOptions
+// (include/pdlp/pdlp_options.h:7-75 there), same fields and defaults, plus
+// the MI355X engine's precision controls at the end.
+#pragma once
+
+#include <cstdint>
+
+namespace pdlp {
+
+enum class Precision {
+    Fp64,   // matrix, iterates and products in FP64
+    Fp32    // FP32 matrix/iterates/products, FP64 checks and a final FP64 phase
+};
+
+struct PdlpOptions {
+    std::int64_t iterationLimit = 200000;
+    double timeLimitSeconds = 0.0;
+
+    double primalTolerance = 1e-6;
+    double dualTolerance = 1e-6;
+    double gapTolerance = 1e-6;
+
+    int terminationCheckFrequency = 100;
+    int powerIterationCount = 100;
+
+    // Host threads (0 = hardware concurrency).  The device ignores them.
+    int threadCount = 0;
+    std::int64_t parallelNonzeroThreshold = 20000;
+
+    double initialStepSafety = 0.90;
+    double minimumStepScale = 1e-6;
+    double maximumPrimalWeight = 1e6;
+    double minimumPrimalWeight = 1e-6;
+
+    bool useRuizScaling = true;
+    int ruizIterations = 10;
+
+    bool detectInfeasibility = true;
+    double infeasibilityTolerance = 1e-8;
+
+    bool useDiagonalPreconditioning = true;
+
+    bool useAdaptiveLinesearch = true;
+    int maximumStepTrials = 60;
+    bool useAveraging = true;
+    bool useRestarts = true;
+    bool useFeasibilityPolishing = true;
+
+    int minimumRestartIterations = 64;
+    int maximumRestartIterations = 100000;
+    double restartSufficientDecay = 0.20;
+    double restartNecessaryDecay = 0.80;
+    double restartArtificialFraction = 0.36;
+
+    double initialPrimalWeight = 1.0;
+    double primalWeightSmoothing = 0.50;
+
+    int polishingIterations = 5000;
+
+    // ---- MI355X engine ----
+    Precision precision = Precision::Fp64;
+    // The FP32 phase hands over to FP64 once the best KKT score is within
+    // this factor of the tolerance, or after this many checks without progress.
+    double fp32HandoffFactor = 10.0;
+    int fp32StallChecks = 20;
+    // HIP device ordinal.
+    int device = 0;
+};
+
+}  // namespace pdlp

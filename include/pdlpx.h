@@ -1,0 +1,212 @@
+/*
+ * pdlpx.h -- C ABI of the MI355X PDLP engine.
+ *
+ * The C++ drop-in (include/pdlp/pdlp_solver.h, pdlp::PdlpSolver::solve) is
+ * implemented on top of these entry points, and any other host language
+ * (Python ctypes, cgo, JNI) binds them directly.  All structs are plain old
+ * data; every pointer is borrowed for the duration of the call; every entry
+ * point returns 0 on success or a negative PDLPX_E_* code and never throws.
+ *
+ * Reference interfaces replaced (pdlp_engine/ of the reference repository):
+ *   pdlpx_problem  <- pdlp::CompiledLp            include/pdlp/compiled_lp.h:17-63
+ *                     pdlp::SparseMatrix storage   include/pdlp/sparse_matrix.h:52-135
+ *   pdlpx_options  <- pdlp::PdlpOptions           include/pdlp/pdlp_options.h:7-75
+ *   pdlpx_summary  <- pdlp::PdlpResult scalars    include/pdlp/pdlp_result.h:10-62
+ *   pdlpx_solve    <- pdlp::PdlpSolver::solve     include/pdlp/pdlp_solver.h:9-15,
+ *                                                  src/pdlp_solver.cpp:116-548
+ *   pdlpx_spmv     <- SparseMatrix::multiply / transposeMultiply
+ *                                                  src/sparse_matrix.cpp:276-419
+ *   pdlpx_scaling  <- RuizScaler::equilibrate + Preconditioner::compute
+ *                                                  src/scaling.cpp:55-136,
+ *                                                  src/preconditioner.cpp:31-80
+ *   pdlpx_solve_batch has no single reference counterpart: it is the
+ *                     reference solve applied to many independent LPs.
+ */
+#ifndef PDLPX_H
+#define PDLPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDLPX_OK 0
+#define PDLPX_E_INVALID (-1)   /* bad argument or problem; message in pdlpx_last_error */
+#define PDLPX_E_HIP (-2)       /* a HIP runtime call failed */
+#define PDLPX_E_NOMEM (-3)     /* device allocation refused */
+#define PDLPX_E_STATE (-4)     /* call out of order (e.g. solve before load) */
+
+/* PdlpStatus, same order as the reference enum (pdlp_result.h:10-31). */
+enum pdlpx_status {
+    PDLPX_OPTIMAL = 0,
+    PDLPX_INFEASIBLE = 1,
+    PDLPX_UNBOUNDED = 2,
+    PDLPX_ITERATION_LIMIT = 3,
+    PDLPX_TIME_LIMIT = 4,
+    PDLPX_NUMERICAL_FAILURE = 5,
+    PDLPX_INVALID_PROBLEM = 6
+};
+
+/* Arithmetic of the PDHG iterations. */
+enum pdlpx_precision {
+    PDLPX_FP64 = 0,   /* matrix, iterates and SpMV in FP64 (reference arithmetic) */
+    PDLPX_FP32 = 1    /* FP32 matrix/iterates/SpMV; FP64 KKT checks, gap sums,
+                         Ruiz and a final FP64 PDHG phase */
+};
+
+/*
+ * minimize c^T x + offset  s.t.  row_lower <= A x <= row_upper,
+ *                                var_lower <= x <= var_upper.
+ * Infinite bounds are IEEE infinities.  The CSR and CSC copies must describe
+ * the same matrix with rows sorted inside each CSC column and columns sorted
+ * inside each CSR row, duplicates merged and exact zeros dropped -- the form
+ * SparseMatrix::fromTriplets produces.
+ */
+typedef struct pdlpx_problem {
+    int32_t rows;
+    int32_t cols;
+    int64_t nnz;
+    const int64_t* csr_row_start;   /* rows + 1 */
+    const int32_t* csr_col;         /* nnz */
+    const double* csr_val;          /* nnz */
+    const int64_t* csc_col_start;   /* cols + 1 */
+    const int32_t* csc_row;         /* nnz */
+    const double* csc_val;          /* nnz */
+    const double* objective;        /* cols */
+    double objective_offset;
+    const double* row_lower;        /* rows */
+    const double* row_upper;        /* rows */
+    const double* var_lower;        /* cols */
+    const double* var_upper;        /* cols */
+} pdlpx_problem;
+
+/* Field-for-field mirror of pdlp::PdlpOptions, plus engine extensions at the end. */
+typedef struct pdlpx_options {
+    int64_t iteration_limit;
+    double time_limit_seconds;
+    double primal_tolerance;
+    double dual_tolerance;
+    double gap_tolerance;
+    int32_t termination_check_frequency;
+    int32_t power_iteration_count;
+    int32_t thread_count;                 /* host threads; the device ignores it */
+    int32_t ruiz_iterations;
+    int64_t parallel_nonzero_threshold;   /* accepted for compatibility; unused */
+    double initial_step_safety;
+    double minimum_step_scale;
+    double maximum_primal_weight;
+    double minimum_primal_weight;
+    double infeasibility_tolerance;
+    double restart_sufficient_decay;
+    double restart_necessary_decay;
+    double restart_artificial_fraction;
+    double initial_primal_weight;
+    double primal_weight_smoothing;
+    int32_t use_ruiz_scaling;
+    int32_t detect_infeasibility;
+    int32_t use_diagonal_preconditioning;
+    int32_t use_adaptive_linesearch;
+    int32_t maximum_step_trials;
+    int32_t use_averaging;
+    int32_t use_restarts;
+    int32_t use_feasibility_polishing;
+    int32_t minimum_restart_iterations;
+    int32_t maximum_restart_iterations;
+    int32_t polishing_iterations;
+    /* ---- engine extensions (no reference counterpart) ---- */
+    int32_t precision;                    /* enum pdlpx_precision */
+    double fp32_handoff_factor;           /* FP32 phase hands over to FP64 once the
+                                             best KKT score <= factor * tolerance */
+    int32_t fp32_stall_checks;            /* ... or after this many checks without
+                                             improving the best score */
+    int32_t reserved;
+} pdlpx_options;
+
+typedef struct pdlpx_summary {
+    int32_t status;                       /* enum pdlpx_status */
+    int32_t restart_count;
+    char message[160];
+    double primal_objective;
+    double dual_objective;
+    double primal_residual;
+    double dual_residual;
+    double relative_gap;
+    int64_t iterations;
+    int64_t step_trials;
+    double solve_time_seconds;
+    double final_step_size;
+    double static_step_bound;
+    double final_primal_weight;
+    int32_t has_dual_ray;                 /* ray_out (rows) written */
+    int32_t has_primal_ray;               /* ray_out (cols) written */
+    /* ---- engine diagnostics ---- */
+    int64_t fp32_iterations;              /* iterations run in the FP32 phase */
+    double device_loop_seconds;           /* hipEvent time of the iteration loop */
+    double setup_seconds;                 /* scaling + preconditioner on device */
+    int32_t termination_checks;
+    int32_t reserved;
+} pdlpx_summary;
+
+typedef struct pdlpx_ctx pdlpx_ctx;
+
+/* Fills `options` with the reference PdlpOptions defaults (pdlp_options.h:7-75). */
+void pdlpx_default_options(pdlpx_options* options);
+
+/* Creates a context bound to HIP device `device` with its own stream. */
+int pdlpx_create(pdlpx_ctx** ctx, int device);
+void pdlpx_destroy(pdlpx_ctx* ctx);
+const char* pdlpx_last_error(const pdlpx_ctx* ctx);
+
+/*
+ * Validates the problem (CompiledLp::validate semantics) and copies it into
+ * device memory.  `precision` selects which value copies are kept resident.
+ * Replaces any previously loaded problem.
+ */
+int pdlpx_load(pdlpx_ctx* ctx, const pdlpx_problem* problem, int32_t precision);
+
+/*
+ * Solves the loaded LP.  warm_x (cols) / warm_y (rows) are optional host start
+ * points in the caller's coordinates (NULL = the reference's cold start).
+ * x_out (cols) and y_out (rows) receive the returned candidate in the
+ * caller's coordinates and the reference's sign convention (c + A^T y);
+ * ray_out, if non-NULL, must hold max(rows, cols) doubles and receives the
+ * Farkas / improving ray when the status is INFEASIBLE / UNBOUNDED.
+ * An invalid option set yields status INVALID_PROBLEM and return value 0.
+ */
+int pdlpx_solve(pdlpx_ctx* ctx, const pdlpx_options* options,
+                const double* warm_x, const double* warm_y,
+                pdlpx_summary* summary,
+                double* x_out, double* y_out, double* ray_out);
+
+/* y = A x (transpose = 0, x: cols, y: rows) or y = A^T x (transpose = 1) on the
+ * loaded, unscaled matrix, in the loaded precision.  Host vectors. */
+int pdlpx_spmv(pdlpx_ctx* ctx, int32_t transpose, const double* x, double* y);
+
+/* Same on device-resident vectors (for timing); no synchronisation. */
+int pdlpx_spmv_device(pdlpx_ctx* ctx, int32_t transpose, const void* x, void* y);
+
+/*
+ * Runs the solver's set-up on the device and returns its results: Ruiz
+ * row/column scales after `ruiz_iterations` passes and the Pock-Chambolle
+ * primal (T, cols) and dual (Sigma, rows) step scales of the scaled matrix.
+ * Any output pointer may be NULL.
+ */
+int pdlpx_scaling(pdlpx_ctx* ctx, int32_t ruiz_iterations, int32_t use_preconditioning,
+                  double* row_scale, double* col_scale,
+                  double* primal_scale, double* dual_scale);
+
+/*
+ * Solves `count` independent LPs (problems[i]) in one batched device run.
+ * Per-instance outputs: summaries[i]; x_out[i] (cols_i) and y_out[i] (rows_i)
+ * may be NULL.  Options apply to every instance.
+ */
+int pdlpx_solve_batch(pdlpx_ctx* ctx, int32_t count, const pdlpx_problem* problems,
+                      const pdlpx_options* options, pdlpx_summary* summaries,
+                      double* const* x_out, double* const* y_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDLPX_H */

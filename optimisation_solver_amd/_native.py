@@ -1,0 +1,134 @@
+"""ctypes bindings of the engine's C ABI (include/pdlpx.h) and of the
+instance generator (csrc/lpgen.cpp).
+
+The engine library must be built in-tree (``python -m
+optimisation_solver_amd.build``); loading fails loudly otherwise -- there is
+no CPU fallback for any engine entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ENGINE_PATH = PKG / "libpdlp_amd.so"
+LPGEN_PATH = PKG / "liblpgen.so"
+
+i32, i64, f64, vp = C.c_int32, C.c_int64, C.c_double, C.c_void_p
+pf64 = C.POINTER(C.c_double)
+
+PDLPX_OK = 0
+STATUS_NAMES = ["optimal", "infeasible", "unbounded", "iteration_limit", "time_limit",
+                "numerical_failure", "invalid_problem"]
+FP64, FP32 = 0, 1
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("rows", i32), ("cols", i32), ("nnz", i64),
+        ("csr_row_start", vp), ("csr_col", vp), ("csr_val", vp),
+        ("csc_col_start", vp), ("csc_row", vp), ("csc_val", vp),
+        ("objective", vp), ("objective_offset", f64),
+        ("row_lower", vp), ("row_upper", vp), ("var_lower", vp), ("var_upper", vp),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("iteration_limit", i64), ("time_limit_seconds", f64),
+        ("primal_tolerance", f64), ("dual_tolerance", f64), ("gap_tolerance", f64),
+        ("termination_check_frequency", i32), ("power_iteration_count", i32),
+        ("thread_count", i32), ("ruiz_iterations", i32),
+        ("parallel_nonzero_threshold", i64),
+        ("initial_step_safety", f64), ("minimum_step_scale", f64),
+        ("maximum_primal_weight", f64), ("minimum_primal_weight", f64),
+        ("infeasibility_tolerance", f64),
+        ("restart_sufficient_decay", f64), ("restart_necessary_decay", f64),
+        ("restart_artificial_fraction", f64),
+        ("initial_primal_weight", f64), ("primal_weight_smoothing", f64),
+        ("use_ruiz_scaling", i32), ("detect_infeasibility", i32),
+        ("use_diagonal_preconditioning", i32), ("use_adaptive_linesearch", i32),
+        ("maximum_step_trials", i32), ("use_averaging", i32), ("use_restarts", i32),
+        ("use_feasibility_polishing", i32), ("minimum_restart_iterations", i32),
+        ("maximum_restart_iterations", i32), ("polishing_iterations", i32),
+        ("precision", i32), ("fp32_handoff_factor", f64), ("fp32_stall_checks", i32),
+        ("reserved", i32),
+    ]
+
+
+class Summary(C.Structure):
+    _fields_ = [
+        ("status", i32), ("restart_count", i32), ("message", C.c_char * 160),
+        ("primal_objective", f64), ("dual_objective", f64), ("primal_residual", f64),
+        ("dual_residual", f64), ("relative_gap", f64),
+        ("iterations", i64), ("step_trials", i64), ("solve_time_seconds", f64),
+        ("final_step_size", f64), ("static_step_bound", f64), ("final_primal_weight", f64),
+        ("has_dual_ray", i32), ("has_primal_ray", i32),
+        ("fp32_iterations", i64), ("device_loop_seconds", f64), ("setup_seconds", f64),
+        ("termination_checks", i32), ("reserved", i32),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        out["message"] = self.message.decode()
+        out["status_name"] = STATUS_NAMES[self.status]
+        return out
+
+
+_ENGINE = None
+_LPGEN = None
+
+
+def engine() -> C.CDLL:
+    """The engine library; raises if it has not been built."""
+    global _ENGINE
+    if _ENGINE is None:
+        if not ENGINE_PATH.exists():
+            raise RuntimeError(
+                f"{ENGINE_PATH} is missing: build it with `python -m optimisation_solver_amd.build` "
+                "(the engine has no CPU fallback)")
+        lib = C.CDLL(str(ENGINE_PATH))
+        lib.pdlpx_default_options.argtypes = [C.POINTER(Options)]
+        lib.pdlpx_default_options.restype = None
+        lib.pdlpx_create.argtypes = [C.POINTER(vp), C.c_int]
+        lib.pdlpx_destroy.argtypes = [vp]
+        lib.pdlpx_destroy.restype = None
+        lib.pdlpx_last_error.argtypes = [vp]
+        lib.pdlpx_last_error.restype = C.c_char_p
+        lib.pdlpx_load.argtypes = [vp, C.POINTER(Problem), i32]
+        lib.pdlpx_solve.argtypes = [vp, C.POINTER(Options), vp, vp, C.POINTER(Summary), vp, vp, vp]
+        lib.pdlpx_spmv.argtypes = [vp, i32, vp, vp]
+        lib.pdlpx_spmv_device.argtypes = [vp, i32, vp, vp]
+        lib.pdlpx_scaling.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        lib.pdlpx_solve_batch.argtypes = [vp, i32, C.POINTER(Problem), C.POINTER(Options),
+                                          C.POINTER(Summary), vp, vp]
+        _ENGINE = lib
+    return _ENGINE
+
+
+def lpgen() -> C.CDLL:
+    global _LPGEN
+    if _LPGEN is None:
+        if not LPGEN_PATH.exists():
+            raise RuntimeError(f"{LPGEN_PATH} is missing: build with `python -m optimisation_solver_amd.build`")
+        lib = C.CDLL(str(LPGEN_PATH))
+        lib.lpgen_box.argtypes = [i32, i32, i32, C.c_uint64]
+        lib.lpgen_box.restype = vp
+        lib.lpgen_zipf.argtypes = [i32, i32, i32, C.c_uint64]
+        lib.lpgen_zipf.restype = vp
+        lib.lpgen_block.argtypes = [i32, i32, i32, i32, i32, i32, C.c_uint64]
+        lib.lpgen_block.restype = vp
+        lib.lpgen_sizes.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i64)]
+        lib.lpgen_sizes.restype = None
+        lib.lpgen_export.argtypes = [vp] + [vp] * 12
+        lib.lpgen_export.restype = None
+        lib.lpgen_free.argtypes = [vp]
+        lib.lpgen_free.restype = None
+        lib.lpgen_log.argtypes = [f64]
+        lib.lpgen_log.restype = f64
+        lib.lpgen_exp.argtypes = [f64]
+        lib.lpgen_exp.restype = f64
+        lib.lpgen_stream.argtypes = [C.c_uint64, i32, i64, vp]
+        lib.lpgen_stream.restype = None
+        _LPGEN = lib
+    return _LPGEN

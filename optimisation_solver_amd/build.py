@@ -1,0 +1,99 @@
+"""Builds the native libraries in-tree.
+
+  optimisation_solver_amd/libpdlp_amd.so   engine kernels (gfx950) + C ABI +
+                                           C++ drop-in API (hipcc)
+  optimisation_solver_amd/liblpgen.so      instance generators (g++)
+  tests/cpp/test_dropin                    C++ drop-in tests (g++, links the engine)
+  oracle/liboracle.so, oracle/_ref/...     CPU checker (make -C oracle)
+
+Run as ``python -m optimisation_solver_amd.build`` or through
+``__graft_entry__.build()``.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("PDLP_OFFLOAD_ARCH", "gfx950")
+
+# -ffp-contract=off keeps the FP64 epilogues in the reference's rounding;
+# packed FP32 instructions are disabled on purpose for this toolchain.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Wall", "-Wextra", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+ENGINE_SO = PKG / "libpdlp_amd.so"
+LPGEN_SO = PKG / "liblpgen.so"
+DROPIN_TEST = ROOT / "tests" / "cpp" / "test_dropin"
+
+
+def _run(cmd: list[str], cwd: Path | None = None) -> subprocess.Popen:
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    return subprocess.Popen([str(c) for c in cmd], cwd=cwd)
+
+
+def _wait(procs: list[subprocess.Popen]) -> None:
+    bad = [p for p in procs if p.wait() != 0]
+    if bad:
+        raise RuntimeError(f"build step failed: {bad[0].args}")
+
+
+def build_engine() -> list[subprocess.Popen]:
+    tmp = PKG / "_build"
+    tmp.mkdir(exist_ok=True)
+    objs = [tmp / "engine.o", tmp / "pdlp_api.o"]
+    procs = [
+        _run([HIPCC, *HIP_FLAGS, "-c", CSRC / "engine.hip", "-o", objs[0]]),
+        _run([HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", "-x", "c++", "-c",
+              CSRC / "pdlp_api.cpp", "-o", objs[1]]),
+    ]
+    return procs
+
+
+def link_engine() -> subprocess.Popen:
+    tmp = PKG / "_build"
+    return _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", ENGINE_SO,
+                 tmp / "engine.o", tmp / "pdlp_api.o"])
+
+
+def build_lpgen() -> subprocess.Popen:
+    return _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                 "-Wall", "-Wextra", "-o", LPGEN_SO, CSRC / "lpgen.cpp"])
+
+
+def build_dropin_test() -> subprocess.Popen:
+    DROPIN_TEST.parent.mkdir(exist_ok=True)
+    return _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
+                 "-o", DROPIN_TEST, ROOT / "tests" / "cpp" / "test_dropin.cpp",
+                 f"-L{PKG}", "-lpdlp_amd", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../optimisation_solver_amd",
+                 "-pthread"])
+
+
+def build_oracle() -> list[subprocess.Popen]:
+    procs = [_run(["make", "-s", "-B", "-C", ROOT / "oracle"])]
+    if Path("/root/reference/pdlp_engine/src").is_dir():
+        procs.append(_run(["make", "-s", "-B", "-C", ROOT / "oracle", "ref"]))
+    return procs
+
+
+def build_all() -> None:
+    if not Path(HIPCC).exists() and shutil.which("hipcc") is None:
+        raise RuntimeError("hipcc not found")
+    first = build_engine() + [build_lpgen()] + build_oracle()
+    _wait(first)
+    _wait([link_engine()])
+    if (ROOT / "tests" / "cpp" / "test_dropin.cpp").exists():
+        _wait([build_dropin_test()])
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

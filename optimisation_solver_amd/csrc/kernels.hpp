@@ -1,0 +1,694 @@
+// PDLP device kernels for gfx950.  Every kernel here is launched by engine.hip.
+//
+// Naming: "working" coordinates are the Ruiz-scaled problem the iteration runs
+// on; "original" coordinates are the caller's.  T is the iteration precision
+// (double, or float in the mixed-precision phase); every reduction is
+// accumulated in double.
+#pragma once
+
+#include "device_common.hpp"
+
+#include <cmath>
+#include <cstdint>
+
+namespace pdlpx_dev {
+
+// Device-resident controller state.  The trial / decide / average kernels of
+// one PDHG iteration read and write it, so the adaptive linesearch is decided
+// on the device without a host round trip (step_controller.cpp:30-68,
+// pdlp_solver.cpp:292-348).
+struct DevState {
+    double eta;          // the step controller's eta
+    double eta_trial;    // eta the next trial uses (== eta outside polishing)
+    double omega;        // primal weight
+    double max_safe;
+    double min_step;     // minimumStepScale * max_safe
+    double avg_total;    // accumulated averaging weight
+    double avg_frac;     // blend fraction of the pending averaging update
+    double pm, dm, inter;
+    int64_t iteration;   // committed iterations
+    int64_t target;      // trials stop once iteration reaches this
+    int64_t trials;
+    int32_t parity;      // which of the two buffers holds the current iterate
+    int32_t done;
+    int32_t fail;        // 1 = non-finite trial, 2 = linesearch exhausted
+    int32_t attempts;    // rejected trials in the current iteration
+    int32_t avg_pending;
+    int32_t adaptive;
+    int32_t averaging;
+    int32_t max_trials;
+};
+
+template <typename T>
+struct ColData {   // per-column data of the working problem
+    const T* c;
+    const T* lo;
+    const T* hi;
+    const T* tau;
+    const T* tau_inv;
+};
+
+template <typename T>
+struct RowData {   // per-row data of the working problem
+    const T* lo;
+    const T* hi;
+    const T* sig;
+    const T* sig_inv;
+};
+
+template <typename T>
+struct Iterates {  // double-buffered state; index `parity` is the current one
+    T* x[2];
+    T* y[2];
+    T* ax[2];
+    T* avg_x;
+    T* avg_y;
+};
+
+template <typename T>
+__device__ inline T floor_step();
+template <>
+__device__ inline double floor_step<double>() { return 1e-300; }   // pdhg_kernel.cpp:13
+template <>
+__device__ inline float floor_step<float>() { return 1e-37f; }
+
+// Primal half of a trial (pdhg_kernel.cpp:47-104): per column j of the CSC
+// orientation, g_j = c_j + (A^T y)_j and x'_j = clip(x_j - (eta/omega) T_j g_j);
+// accumulates sum dx_j^2 / T_j.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_primal(Orient<T> at, ColData<T> cd, Iterates<T> it,
+                                                     const DevState* __restrict__ st,
+                                                     double* __restrict__ part) {
+    if (st->done) return;
+    __shared__ T lds[kTileNnz];
+    __shared__ T shT[kWaves];
+    __shared__ double sh[kWaves];
+    const int p = st->parity;
+    const T mul = static_cast<T>(st->eta_trial / st->omega);
+    const T* __restrict__ x = it.x[p];
+    const T* __restrict__ y = it.y[p];
+    T* __restrict__ xt = it.x[p ^ 1];
+    double acc = 0.0;
+    auto epi = [&](int j, T s) {
+        const T g = cd.c[j] + s;
+        const T was = x[j];
+        const T now = clip(was - mul * cd.tau[j] * g, cd.lo[j], cd.hi[j]);
+        xt[j] = now;
+        const T d = now - was;
+        acc += static_cast<double>(d * d * cd.tau_inv[j]);
+    };
+    row_block(at, y, lds, shT, epi);
+    double v[1] = {acc};
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Dual half (pdhg_kernel.cpp:109-187): (A x')_i, the extrapolated activity
+// 2 A x' - A x, and the Moreau-form dual prox
+// y' = max(v - s u, 0) + min(v - s l, 0); accumulates sum dy^2 / Sigma and
+// sum dy * (A dx).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dual(Orient<T> a, RowData<T> rd, Iterates<T> it,
+                                                   const DevState* __restrict__ st,
+                                                   double* __restrict__ part_d,
+                                                   double* __restrict__ part_i) {
+    if (st->done) return;
+    __shared__ T lds[kTileNnz];
+    __shared__ T shT[kWaves];
+    __shared__ double sh[2 * kWaves];
+    const int p = st->parity;
+    const T mul = static_cast<T>(st->eta_trial * st->omega);
+    const T* __restrict__ xt = it.x[p ^ 1];
+    const T* __restrict__ y = it.y[p];
+    const T* __restrict__ ax = it.ax[p];
+    T* __restrict__ yt = it.y[p ^ 1];
+    T* __restrict__ axt = it.ax[p ^ 1];
+    double mov = 0.0, inter = 0.0;
+    auto epi = [&](int i, T act) {
+        const T dact = act - ax[i];
+        const T ext = act + dact;
+        const T step = smax(mul * rd.sig[i], floor_step<T>());
+        const T was = y[i];
+        const T v = was + step * ext;
+        const T now = smax(v - step * rd.hi[i], T(0)) + smin(v - step * rd.lo[i], T(0));
+        yt[i] = now;
+        axt[i] = act;
+        const T d = now - was;
+        mov += static_cast<double>(d * d * rd.sig_inv[i]);
+        inter += static_cast<double>(d) * static_cast<double>(dact);
+    };
+    row_block(a, xt, lds, shT, epi);
+    double v[2] = {mov, inter};
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part_d[blockIdx.x] = v[0];
+        part_i[blockIdx.x] = v[1];
+    }
+}
+
+template <int NT, int N>
+__device__ inline void group_sums(double (&v)[N], double* sh) {
+    constexpr int W = NT / 64;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        const double s = wave_sum(v[q]);
+        if (lane == 0) sh[q * W + w] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            double s = 0.0;
+            for (int i = 0; i < W; ++i) s += sh[q * W + i];
+            v[q] = s;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ inline double dclamp(double v, double lo, double hi) {   // std::clamp
+    return v < lo ? lo : (hi < v ? hi : v);
+}
+
+// Linesearch decision and commit (step_controller.cpp:30-68,
+// pdlp_solver.cpp:292-348, iterate_average.cpp:205-247).
+__global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* st, const double* __restrict__ pp,
+                                                            int np, const double* __restrict__ pd,
+                                                            const double* __restrict__ pi, int nd) {
+    __shared__ double sh[3 * (kReduceThreads / 64)];
+    __shared__ int stop;
+    if (threadIdx.x == 0) {
+        st->avg_pending = 0;
+        stop = st->done;
+    }
+    __syncthreads();
+    if (stop) return;
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < np; i += kReduceThreads) v[0] += pp[i];
+    for (int i = threadIdx.x; i < nd; i += kReduceThreads) {
+        v[1] += pd[i];
+        v[2] += pi[i];
+    }
+    group_sums<kReduceThreads, 3>(v, sh);
+    if (threadIdx.x != 0) return;
+    const double pm = v[0], dm = v[1], inter = v[2];
+    st->pm = pm;
+    st->dm = dm;
+    st->inter = inter;
+    st->trials += 1;
+    if (!(isfinite(pm) && isfinite(dm) && isfinite(inter))) {
+        st->fail = 1;
+        st->done = 1;
+        return;
+    }
+    bool ok = true;
+    if (st->adaptive) {
+        const double omega = st->omega;
+        const double mov = omega * pm + dm / omega;
+        const double cpl = fabs(inter);
+        double limit = INFINITY;
+        if (cpl > 0.0 && mov > 0.0) limit = mov / (2.0 * cpl);
+        const double k = static_cast<double>(st->iteration) + 2.0;
+        const double shrink = 1.0 - pow(k, -0.3);
+        const double grow = 1.0 + pow(k, -0.6);
+        ok = st->eta <= limit;
+        double next = smin(shrink * limit, grow * st->eta);
+        next = dclamp(next, st->min_step, 1e12);
+        if (!isfinite(next) || next <= 0.0) next = st->eta;
+        st->eta = next;
+        st->eta_trial = next;
+    }
+    if (ok) {
+        st->parity ^= 1;
+        st->iteration += 1;
+        st->attempts = 0;
+        if (st->averaging) {
+            const double w = st->eta_trial;
+            if (w > 0.0) {
+                const double total = st->avg_total + w;
+                st->avg_frac = w / total;
+                st->avg_total = total;
+                st->avg_pending = 1;
+            }
+        }
+        if (st->iteration >= st->target) st->done = 1;
+    } else {
+        st->attempts += 1;
+        if (st->attempts >= st->max_trials) {
+            st->fail = 2;
+            st->done = 1;
+        }
+    }
+}
+
+// avg += f * (x - avg) over the freshly committed iterate (iterate_average.cpp:14-24).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict__ st, Iterates<T> it,
+                                                      int n, int m) {
+    if (!st->avg_pending) return;
+    const T f = static_cast<T>(st->avg_frac);
+    const int p = st->parity;
+    const int stride = gridDim.x * kThreads;
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
+        if (q < n) {
+            const T a = it.avg_x[q];
+            it.avg_x[q] = a + f * (it.x[p][q] - a);
+        } else {
+            const int i = q - n;
+            const T a = it.avg_y[i];
+            it.avg_y[i] = a + f * (it.y[p][i] - a);
+        }
+    }
+}
+
+// Plain row sums out = M v (refreshActivity, debug products).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_spmv(Orient<T> M, const T* __restrict__ v, T* __restrict__ out) {
+    __shared__ T lds[kTileNnz];
+    __shared__ T shT[kWaves];
+    auto epi = [&](int r, T s) { out[r] = s; };
+    row_block(M, v, lds, shT, epi);
+}
+
+// Working -> original coordinates: xo = cs * x, yo = rs * y (scaling.cpp:23-37).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_to_original(int n, int m, const double* __restrict__ cs,
+                                                          const double* __restrict__ rs,
+                                                          const T* __restrict__ x, const T* __restrict__ y,
+                                                          double* __restrict__ xo, double* __restrict__ yo) {
+    const int stride = gridDim.x * kThreads;
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
+        if (q < n) xo[q] = cs[q] * static_cast<double>(x[q]);
+        else yo[q - n] = rs[q - n] * static_cast<double>(y[q - n]);
+    }
+}
+
+// Original -> working coordinates: x = xo / cs, y = yo / rs (scaling.cpp:39-53).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_to_working(int n, int m, const double* __restrict__ cs,
+                                                         const double* __restrict__ rs,
+                                                         const double* __restrict__ xo,
+                                                         const double* __restrict__ yo,
+                                                         T* __restrict__ x, T* __restrict__ y) {
+    const int stride = gridDim.x * kThreads;
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
+        if (q < n) x[q] = static_cast<T>(xo[q] / cs[q]);
+        else y[q - n] = static_cast<T>(yo[q - n] / rs[q - n]);
+    }
+}
+
+// Termination row pass for two candidates (termination.cpp:66-135) on the
+// original matrix: violation^2, support and dual-cone drift^2 per candidate.
+__global__ __launch_bounds__(kThreads) void k_term_rows(Orient<double> a, const double* __restrict__ x0,
+                                                        const double* __restrict__ x1,
+                                                        const double* __restrict__ y0,
+                                                        const double* __restrict__ y1,
+                                                        const double* __restrict__ lo,
+                                                        const double* __restrict__ hi,
+                                                        double* __restrict__ part) {
+    __shared__ double lds[2 * kTileNnz];
+    __shared__ double shT[kWaves];
+    __shared__ double sh[6 * kWaves];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    auto one = [&](int i, double act, double yv, double* o) {
+        const double l = lo[i], u = hi[i];
+        const double e = act - clip(act, l, u);
+        o[0] += e * e;
+        double q = yv;
+        if (!isfinite(u)) q = smin(q, 0.0);
+        if (!isfinite(l)) q = smax(q, 0.0);
+        const double dr = yv - q;
+        o[2] += dr * dr;
+        if (q > 0.0) o[1] += q * u;
+        else if (q < 0.0) o[1] += q * l;
+    };
+    auto epi = [&](int i, double s0, double s1) {
+        one(i, s0, y0[i], acc);
+        one(i, s1, y1[i], acc + 3);
+    };
+    row_block2(a, x0, x1, lds, shT, epi);
+    block_sums<6>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) part[q * a.nblk + blockIdx.x] = acc[q];
+}
+
+// Termination column pass for two candidates (termination.cpp:140-209):
+// c^T x, the reduced-cost infimum over the box and dual drift^2.
+__global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const double* __restrict__ y0,
+                                                        const double* __restrict__ y1,
+                                                        const double* __restrict__ x0,
+                                                        const double* __restrict__ x1,
+                                                        const double* __restrict__ c,
+                                                        const double* __restrict__ lo,
+                                                        const double* __restrict__ hi,
+                                                        double* __restrict__ part) {
+    __shared__ double lds[2 * kTileNnz];
+    __shared__ double shT[kWaves];
+    __shared__ double sh[6 * kWaves];
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    auto one = [&](int j, double s, double xv, double* o) {
+        const double rc = c[j] + s;
+        const double l = lo[j], u = hi[j];
+        o[0] += c[j] * xv;
+        double q = rc;
+        if (!isfinite(l)) q = smin(q, 0.0);
+        if (!isfinite(u)) q = smax(q, 0.0);
+        const double dr = rc - q;
+        o[2] += dr * dr;
+        if (q > 0.0) o[1] += q * l;
+        else if (q < 0.0) o[1] += q * u;
+    };
+    auto epi = [&](int j, double s0, double s1) {
+        one(j, s0, x0[j], acc);
+        one(j, s1, x1[j], acc + 3);
+    };
+    row_block2(at, y0, y1, lds, shT, epi);
+    block_sums<6>(acc, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) part[q * at.nblk + blockIdx.x] = acc[q];
+}
+
+// Fixed-order reduction of up to 16 partial arrays, one workgroup each.
+struct ReduceJob {
+    const double* src[16];
+    int32_t count[16];
+    double* dst[16];
+};
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(ReduceJob job) {
+    __shared__ double sh[kReduceThreads / 64];
+    const int q = blockIdx.x;
+    const double* __restrict__ s = job.src[q];
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < job.count[q]; i += kReduceThreads) v[0] += s[i];
+    group_sums<kReduceThreads, 1>(v, sh);
+    if (threadIdx.x == 0) *job.dst[q] = v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Set-up kernels: Ruiz (scaling.cpp:55-136), Pock-Chambolle
+// (preconditioner.cpp:31-80), start point (pdlp_solver.cpp:30-41,168-176).
+// The maxima are order independent and the sums run in the reference's order
+// (CSR order for rows, CSC = increasing-row order for columns), so the scale
+// vectors are bitwise those of the reference.
+// ---------------------------------------------------------------------------
+__global__ void k_ruiz_max(int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                           const double* __restrict__ val, const double* __restrict__ own,
+                           const double* __restrict__ other, bool ownIsRow, double* __restrict__ out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const double f = own[r];
+    double best = 0.0;
+    for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+        // |a| * rowScale * colScale, in that association (scaling.cpp:85-86)
+        const double mag = ownIsRow ? fabs(val[k]) * f * other[idx[k]] : fabs(val[k]) * other[idx[k]] * f;
+        best = smax(best, mag);
+    }
+    out[r] = best;
+}
+
+__global__ void k_ruiz_update(int n, double* __restrict__ scale, const double* __restrict__ mx) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    if (mx[r] > 1e-30) scale[r] /= sqrt(mx[r]);
+}
+
+// Scaled values a * (own * other) (sparse_matrix.cpp:243-274), in both
+// precisions.
+__global__ void k_scale_values(int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
+                               const double* __restrict__ val, const double* __restrict__ own,
+                               const double* __restrict__ other, double* __restrict__ out64,
+                               float* __restrict__ out32) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const double f = own[r];
+    for (int k = ptr[r]; k < ptr[r + 1]; ++k) {
+        const double v = val[k] * (f * other[idx[k]]);
+        out64[k] = v;
+        if (out32) out32[k] = static_cast<float>(v);
+    }
+}
+
+// Sum of |a| along each row of one orientation -> step scale 1/sum (or 1).
+__global__ void k_abs_sums(int nrows, const int32_t* __restrict__ ptr, const double* __restrict__ val,
+                           double* __restrict__ scale, double* __restrict__ inv) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    double acc = 0.0;
+    for (int k = ptr[r]; k < ptr[r + 1]; ++k) acc += fabs(val[k]);
+    const double s = acc > 1e-12 ? 1.0 / acc : 1.0;
+    scale[r] = s;
+    inv[r] = 1.0 / s;
+}
+
+__device__ inline double scale_bound(double b, double f) { return isfinite(b) ? b * f : b; }
+
+// Scaled column data: c * cs, bounds * (1/cs) (scaling.cpp:115-124).
+__global__ void k_scale_cols(int n, const double* __restrict__ cs, const double* __restrict__ c,
+                             const double* __restrict__ lo, const double* __restrict__ hi,
+                             double* __restrict__ sc, double* __restrict__ slo, double* __restrict__ shi) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double f = cs[j];
+    sc[j] = c[j] * f;
+    slo[j] = scale_bound(lo[j], 1.0 / f);
+    shi[j] = scale_bound(hi[j], 1.0 / f);
+}
+
+// Scaled row bounds: bounds * rs (scaling.cpp:126-133).
+__global__ void k_scale_rows(int m, const double* __restrict__ rs, const double* __restrict__ lo,
+                             const double* __restrict__ hi, double* __restrict__ slo,
+                             double* __restrict__ shi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    slo[i] = scale_bound(lo[i], rs[i]);
+    shi[i] = scale_bound(hi[i], rs[i]);
+}
+
+template <typename D, typename S>
+__global__ void k_convert(int64_t n, const S* __restrict__ src, D* __restrict__ dst) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride)
+        dst[q] = static_cast<D>(src[q]);
+}
+
+template <typename T>
+__global__ void k_fill(int64_t n, T* __restrict__ dst, T value) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = value;
+}
+
+// Start point in working coordinates (pdlp_solver.cpp:30-41): x from the
+// bounds, y = 0; or, with a warm start (original coordinates), the warm point
+// mapped to working coordinates and projected into the box / dual cone.
+__global__ void k_start(int n, int m, const double* __restrict__ lo, const double* __restrict__ hi,
+                        const double* __restrict__ rlo, const double* __restrict__ rhi,
+                        const double* __restrict__ cs, const double* __restrict__ rs,
+                        const double* __restrict__ wx, const double* __restrict__ wy,
+                        double* __restrict__ x, double* __restrict__ y) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
+        const double l = lo[q], u = hi[q];
+        if (wx) {
+            x[q] = clip(wx[q] / cs[q], l, u);
+        } else {
+            double v = 0.0;
+            if (l <= 0.0 && 0.0 <= u) v = 0.0;
+            else if (isfinite(l)) v = l;
+            else if (isfinite(u)) v = u;
+            x[q] = v;
+        }
+    } else if (q < n + m) {
+        const int i = q - n;
+        double v = 0.0;
+        if (wy) {
+            v = wy[i] / rs[i];
+            if (!isfinite(rhi[i])) v = smin(v, 0.0);
+            if (!isfinite(rlo[i])) v = smax(v, 0.0);
+        }
+        y[i] = v;
+    }
+}
+
+// Streaming partial sums: sum (a-b)^2 over two vector pairs (restart
+// displacement, pdlp_solver.cpp:81-91,456-459).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sqdist(int n, const T* __restrict__ a0, const T* __restrict__ b0,
+                                                     int m, const T* __restrict__ a1, const T* __restrict__ b1,
+                                                     double* __restrict__ part) {
+    __shared__ double sh[2 * kWaves];
+    double v[2] = {0.0, 0.0};
+    const int stride = gridDim.x * kThreads;
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
+        if (q < n) {
+            const double d = static_cast<double>(a0[q]) - static_cast<double>(b0[q]);
+            v[0] += d * d;
+        } else {
+            const double d = static_cast<double>(a1[q - n]) - static_cast<double>(b1[q - n]);
+            v[1] += d * d;
+        }
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// Ray directions in original coordinates: rx = cs * (x - anchor_x),
+// ry = rs * (y - anchor_y) (pdlp_solver.cpp:378-390), with their squared norms.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_ray_dir(int n, int m, const double* __restrict__ cs,
+                                                      const double* __restrict__ rs, const T* __restrict__ x,
+                                                      const T* __restrict__ ax, const T* __restrict__ y,
+                                                      const T* __restrict__ ay, double* __restrict__ rx,
+                                                      double* __restrict__ ry, double* __restrict__ part) {
+    __shared__ double sh[2 * kWaves];
+    double v[2] = {0.0, 0.0};
+    const int stride = gridDim.x * kThreads;
+    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
+        if (q < n) {
+            const double d = cs[q] * (static_cast<double>(x[q]) - static_cast<double>(ax[q]));
+            rx[q] = d;
+            v[0] += d * d;
+        } else {
+            const int i = q - n;
+            const double d = rs[i] * (static_cast<double>(y[i]) - static_cast<double>(ay[i]));
+            ry[i] = d;
+            v[1] += d * d;
+        }
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// Dual ray, row terms (infeasibility.cpp:92-115): u = ry / ||ry||, drift and
+// support over the row bounds.  norms[1] holds ||ry||^2.
+__global__ __launch_bounds__(kThreads) void k_farkas_rows(int m, const double* __restrict__ ry,
+                                                          const double* __restrict__ norms,
+                                                          const double* __restrict__ lo,
+                                                          const double* __restrict__ hi,
+                                                          double* __restrict__ u, double* __restrict__ part) {
+    __shared__ double sh[2 * kWaves];
+    const double inv = 1.0 / sqrt(norms[1]);
+    double v[2] = {0.0, 0.0};   // violation^2, support
+    const int stride = gridDim.x * kThreads;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < m; i += stride) {
+        const double val = ry[i] * inv;
+        u[i] = val;
+        double q = val;
+        if (!isfinite(hi[i])) q = smin(q, 0.0);
+        if (!isfinite(lo[i])) q = smax(q, 0.0);
+        const double dr = val - q;
+        v[0] += dr * dr;
+        if (q > 0.0) v[1] += q * hi[i];
+        else if (q < 0.0) v[1] += q * lo[i];
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// Dual ray, column terms (infeasibility.cpp:62-90): A^T u, its cone drift and
+// the box infimum.
+__global__ __launch_bounds__(kThreads) void k_farkas_cols(Orient<double> at, const double* __restrict__ u,
+                                                          const double* __restrict__ lo,
+                                                          const double* __restrict__ hi,
+                                                          double* __restrict__ part) {
+    __shared__ double lds[kTileNnz];
+    __shared__ double shT[kWaves];
+    __shared__ double sh[2 * kWaves];
+    double v[2] = {0.0, 0.0};   // violation^2, infimum
+    auto epi = [&](int j, double rc) {
+        double q = rc;
+        if (!isfinite(lo[j])) q = smin(q, 0.0);
+        if (!isfinite(hi[j])) q = smax(q, 0.0);
+        const double dr = rc - q;
+        v[0] += dr * dr;
+        if (q > 0.0) v[1] += q * lo[j];
+        else if (q < 0.0) v[1] += q * hi[j];
+    };
+    row_block(at, u, lds, shT, epi);
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[at.nblk + blockIdx.x] = v[1];
+    }
+}
+
+// Primal ray, column terms (infeasibility.cpp:142-161): u = rx / ||rx||
+// projected onto the box's recession cone; drift and c^T d.  norms[0] holds
+// ||rx||^2.
+__global__ __launch_bounds__(kThreads) void k_ray_cols(int n, const double* __restrict__ rx,
+                                                       const double* __restrict__ norms,
+                                                       const double* __restrict__ c,
+                                                       const double* __restrict__ lo,
+                                                       const double* __restrict__ hi,
+                                                       double* __restrict__ proj, double* __restrict__ part) {
+    __shared__ double sh[2 * kWaves];
+    const double inv = 1.0 / sqrt(norms[0]);
+    double v[2] = {0.0, 0.0};   // violation^2, objective rate
+    const int stride = gridDim.x * kThreads;
+    for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += stride) {
+        const double val = rx[j] * inv;
+        double q = val;
+        if (isfinite(hi[j])) q = smin(q, 0.0);
+        if (isfinite(lo[j])) q = smax(q, 0.0);
+        const double dr = val - q;
+        v[0] += dr * dr;
+        proj[j] = q;
+        v[1] += c[j] * q;
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+// Primal ray, row terms (infeasibility.cpp:163-178): A d inside the row
+// bounds' recession cone.
+__global__ __launch_bounds__(kThreads) void k_ray_rows(Orient<double> a, const double* __restrict__ proj,
+                                                       const double* __restrict__ lo,
+                                                       const double* __restrict__ hi,
+                                                       double* __restrict__ part) {
+    __shared__ double lds[kTileNnz];
+    __shared__ double shT[kWaves];
+    __shared__ double sh[kWaves];
+    double v[1] = {0.0};
+    auto epi = [&](int i, double act) {
+        double q = act;
+        if (isfinite(hi[i])) q = smin(q, 0.0);
+        if (isfinite(lo[i])) q = smax(q, 0.0);
+        const double dr = act - q;
+        v[0] += dr * dr;
+    };
+    row_block(a, proj, lds, shT, epi);
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Power iteration helpers (preconditioner.cpp:82-140): dot product partials.
+__global__ __launch_bounds__(kThreads) void k_dot(int n, const double* __restrict__ a,
+                                                  const double* __restrict__ b, double* __restrict__ part) {
+    __shared__ double sh[kWaves];
+    double v[1] = {0.0};
+    const int stride = gridDim.x * kThreads;
+    for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += stride) v[0] += a[j] * b[j];
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+__global__ void k_scale_vec(int n, double* __restrict__ v, double s) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) v[j] *= s;
+}
+
+}  // namespace pdlpx_dev

@@ -1,0 +1,393 @@
+// Seeded, portable generators for the benchmark LP families.
+//
+// Every number comes from a xoshiro256** stream and plain IEEE double
+// arithmetic (this file is compiled with -ffp-contract=off, and log/exp are
+// computed here rather than taken from libm), so an instance is byte-identical
+// on every x86-64 host: its SHA-256 is a fixture.
+//
+// Families (m rows, n columns, all feasible by construction from a planted
+// point x0, b = A x0):
+//   box   : k distinct uniform rows per column, values N(0,1); bounds [0,20];
+//           x0 ~ U[0,10]; rows i%5<3 are equalities, the rest l=-inf,
+//           u = b + U[0,1]; c = A^T y0 + s, y0 ~ N(0,1), s ~ U[0,1].
+//   zipf  : column degree ceil(z*10/67.9) with z ~ Zipf(1.5) clipped to
+//           [1,2000] ("literal" keeps z itself); values N(0,1)*rho_i with a
+//           per-row rho_i ~ lognormal(0,2); 10% free columns (j%10==0);
+//           dual-feasible costs: y0 = -|N(0,1)| on <= rows, s = 0 on free
+//           columns, so the LP has a finite optimum.
+//   block : multi-period block-angular LP: P period blocks of the box family
+//           plus linking rows that couple adjacent periods.
+//
+// Reference counterpart: pdlp_engine/bench/bench_pdlp.cpp:21-75 builds a
+// similar planted-point family with std::mt19937; the families above are the
+// ones the benchmark configurations name, with a generator of our own.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+namespace {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+struct Rng {
+    uint64_t s[4];
+    explicit Rng(uint64_t seed) {
+        uint64_t z = seed;
+        for (auto& w : s) {   // splitmix64 seeding
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t t = z;
+            t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+            t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+            w = t ^ (t >> 31);
+        }
+    }
+    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+    uint64_t next() {   // xoshiro256**
+        const uint64_t r = rotl(s[1] * 5, 7) * 9;
+        const uint64_t t = s[1] << 17;
+        s[2] ^= s[0];
+        s[3] ^= s[1];
+        s[1] ^= s[2];
+        s[0] ^= s[3];
+        s[2] ^= t;
+        s[3] = rotl(s[3], 45);
+        return r;
+    }
+    double unit() { return static_cast<double>(next() >> 11) * 0x1.0p-53; }   // [0,1)
+    double uniform(double a, double b) { return a + (b - a) * unit(); }
+    uint32_t below(uint32_t bound) {   // unbiased integer in [0, bound)
+        const uint64_t limit = (~0ull / bound) * bound;
+        uint64_t x;
+        do x = next();
+        while (x >= limit);
+        return static_cast<uint32_t>(x % bound);
+    }
+    bool spare = false;
+    double cached = 0.0;
+    double normal();
+};
+
+// ln(x) for finite x > 0: x = f * 2^e with f in [sqrt(1/2), sqrt(2)),
+// ln x = e ln2 + 2 atanh(t), t = (f-1)/(f+1), |t| < 0.172, series to t^29.
+double detLog(double x) {
+    int e = 0;
+    double f = std::frexp(x, &e);   // f in [0.5, 1)
+    if (f < 0.70710678118654752440) {
+        f *= 2.0;
+        e -= 1;
+    }
+    const double t = (f - 1.0) / (f + 1.0);
+    const double t2 = t * t;
+    double sum = 0.0;
+    for (int k = 29; k >= 1; k -= 2) sum = sum * t2 + 1.0 / k;
+    const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10;
+    return (e * ln2hi + (2.0 * t * sum + e * ln2lo));
+}
+
+// exp(x) for |x| < 700: x = k ln2 + r, |r| <= ln2/2, Taylor to r^24.
+double detExp(double x) {
+    const double ln2hi = 6.93147180369123816490e-01, ln2lo = 1.90821492927058770002e-10;
+    const double k = std::floor(x / 0.69314718055994530942 + 0.5);
+    const double r = (x - k * ln2hi) - k * ln2lo;
+    double sum = 1.0;
+    for (int i = 24; i >= 1; --i) sum = 1.0 + sum * r / i;
+    return std::ldexp(sum, static_cast<int>(k));
+}
+
+double Rng::normal() {   // Marsaglia polar method
+    if (spare) {
+        spare = false;
+        return cached;
+    }
+    double u, v, q;
+    do {
+        u = uniform(-1.0, 1.0);
+        v = uniform(-1.0, 1.0);
+        q = u * u + v * v;
+    } while (q >= 1.0 || q == 0.0);
+    const double f = std::sqrt(-2.0 * detLog(q) / q);
+    cached = v * f;
+    spare = true;
+    return u * f;
+}
+
+// Zipf(s = 1.5) on {1, 2, ...}, clipped at kmax (the tail mass lands on kmax).
+struct Zipf {
+    std::vector<double> cdf;   // cdf[k-1] = P(Z <= k) for k < kmax
+    explicit Zipf(int kmax) {
+        const double zeta15 = 2.6123753486854883433;
+        double acc = 0.0;
+        for (int k = 1; k < kmax; ++k) {
+            acc += 1.0 / (k * std::sqrt(static_cast<double>(k))) / zeta15;
+            cdf.push_back(acc);
+        }
+    }
+    int draw(Rng& g) const {
+        const double u = g.unit();
+        const auto it = std::upper_bound(cdf.begin(), cdf.end(), u);
+        return static_cast<int>(it - cdf.begin()) + 1;   // == kmax when u >= cdf.back()
+    }
+};
+
+struct Instance {
+    int32_t m = 0, n = 0;
+    std::vector<int64_t> rp, cp;
+    std::vector<int32_t> ci, ri;
+    std::vector<double> rv, cv;
+    std::vector<double> c, lo, hi, rlo, rhi, x0;
+    double offset = 0.0;
+};
+
+// Draws k distinct rows in [0, m), sorted.
+void distinctRows(Rng& g, int m, int k, std::vector<int32_t>& out) {
+    out.clear();
+    while (static_cast<int>(out.size()) < k) {
+        const int32_t r = static_cast<int32_t>(g.below(static_cast<uint32_t>(m)));
+        if (std::find(out.begin(), out.end(), r) == out.end()) out.push_back(r);
+    }
+    std::sort(out.begin(), out.end());
+}
+
+// Builds CSR from the CSC already in `ins` (columns row-sorted -> rows column-sorted).
+void finishCsr(Instance& ins) {
+    const int64_t nz = static_cast<int64_t>(ins.cv.size());
+    ins.rp.assign(static_cast<size_t>(ins.m) + 1, 0);
+    for (int64_t k = 0; k < nz; ++k) ins.rp[static_cast<size_t>(ins.ri[k]) + 1]++;
+    for (int i = 0; i < ins.m; ++i) ins.rp[i + 1] += ins.rp[i];
+    ins.ci.resize(static_cast<size_t>(nz));
+    ins.rv.resize(static_cast<size_t>(nz));
+    std::vector<int64_t> fill(ins.rp.begin(), ins.rp.end() - 1);
+    for (int j = 0; j < ins.n; ++j)
+        for (int64_t k = ins.cp[j]; k < ins.cp[j + 1]; ++k) {
+            const int64_t at = fill[ins.ri[k]]++;
+            ins.ci[at] = j;
+            ins.rv[at] = ins.cv[k];
+        }
+}
+
+void rowProduct(const Instance& ins, const std::vector<double>& x, std::vector<double>& y) {
+    y.assign(static_cast<size_t>(ins.m), 0.0);
+    for (int i = 0; i < ins.m; ++i) {
+        double s = 0.0;
+        for (int64_t k = ins.rp[i]; k < ins.rp[i + 1]; ++k) s += ins.rv[k] * x[ins.ci[k]];
+        y[i] = s;
+    }
+}
+
+void colProduct(const Instance& ins, const std::vector<double>& y, std::vector<double>& z) {
+    z.assign(static_cast<size_t>(ins.n), 0.0);
+    for (int j = 0; j < ins.n; ++j) {
+        double s = 0.0;
+        for (int64_t k = ins.cp[j]; k < ins.cp[j + 1]; ++k) s += ins.cv[k] * y[ins.ri[k]];
+        z[j] = s;
+    }
+}
+
+double nonzeroNormal(Rng& g) {
+    double v = g.normal();
+    while (v == 0.0) v = g.normal();
+    return v;
+}
+
+// Planted rows / costs shared by the families: rows i%5<3 equalities, others
+// one-sided with slack U[0,1]; c = A^T y0 + s.
+void plantBoundsAndCosts(Instance& ins, Rng& g, bool dualFeasible, const std::vector<char>& freeCol) {
+    std::vector<double> b;
+    rowProduct(ins, ins.x0, b);
+    ins.rlo.resize(static_cast<size_t>(ins.m));
+    ins.rhi.resize(static_cast<size_t>(ins.m));
+    for (int i = 0; i < ins.m; ++i) {
+        if (i % 5 < 3) {
+            ins.rlo[i] = b[i];
+            ins.rhi[i] = b[i];
+        } else {
+            ins.rlo[i] = -kInf;
+            ins.rhi[i] = b[i] + g.unit();
+        }
+    }
+    std::vector<double> y0(static_cast<size_t>(ins.m));
+    for (int i = 0; i < ins.m; ++i) {
+        const double z = g.normal();
+        y0[i] = (dualFeasible && i % 5 >= 3) ? -std::abs(z) : z;
+    }
+    std::vector<double> aty;
+    colProduct(ins, y0, aty);
+    ins.c.resize(static_cast<size_t>(ins.n));
+    for (int j = 0; j < ins.n; ++j) {
+        const double s = g.unit();
+        ins.c[j] = aty[j] + ((dualFeasible && freeCol[j]) ? 0.0 : s);
+    }
+}
+
+Instance makeBox(int m, int n, int perCol, uint64_t seed) {
+    Instance ins;
+    ins.m = m;
+    ins.n = n;
+    Rng g(seed);
+    std::vector<int32_t> rows;
+    ins.cp.assign(static_cast<size_t>(n) + 1, 0);
+    const int k = std::min(perCol, m);
+    for (int j = 0; j < n; ++j) {
+        distinctRows(g, m, k, rows);
+        for (int32_t r : rows) {
+            ins.ri.push_back(r);
+            ins.cv.push_back(nonzeroNormal(g));
+        }
+        ins.cp[j + 1] = static_cast<int64_t>(ins.ri.size());
+    }
+    finishCsr(ins);
+    ins.x0.resize(static_cast<size_t>(n));
+    for (int j = 0; j < n; ++j) ins.x0[j] = g.uniform(0.0, 10.0);
+    ins.lo.assign(static_cast<size_t>(n), 0.0);
+    ins.hi.assign(static_cast<size_t>(n), 20.0);
+    plantBoundsAndCosts(ins, g, false, std::vector<char>(static_cast<size_t>(n), 0));
+    return ins;
+}
+
+Instance makeZipf(int m, int n, bool literal, uint64_t seed) {
+    Instance ins;
+    ins.m = m;
+    ins.n = n;
+    Rng g(seed);
+    const Zipf zipf(2000);
+    std::vector<double> rho(static_cast<size_t>(m));
+    for (int i = 0; i < m; ++i) rho[i] = detExp(2.0 * g.normal());
+    std::vector<int32_t> rows;
+    ins.cp.assign(static_cast<size_t>(n) + 1, 0);
+    for (int j = 0; j < n; ++j) {
+        const int z = zipf.draw(g);
+        int k = literal ? z : static_cast<int>(std::ceil(z * 10.0 / 67.9));
+        k = std::max(1, std::min(k, m));
+        distinctRows(g, m, k, rows);
+        for (int32_t r : rows) {
+            ins.ri.push_back(r);
+            ins.cv.push_back(nonzeroNormal(g) * rho[r]);
+        }
+        ins.cp[j + 1] = static_cast<int64_t>(ins.ri.size());
+    }
+    finishCsr(ins);
+    std::vector<char> freeCol(static_cast<size_t>(n), 0);
+    ins.x0.resize(static_cast<size_t>(n));
+    ins.lo.resize(static_cast<size_t>(n));
+    ins.hi.resize(static_cast<size_t>(n));
+    for (int j = 0; j < n; ++j) {
+        ins.x0[j] = g.uniform(0.0, 10.0);
+        freeCol[j] = (j % 10 == 0) ? 1 : 0;
+        ins.lo[j] = freeCol[j] ? -kInf : 0.0;
+        ins.hi[j] = freeCol[j] ? kInf : 20.0;
+    }
+    plantBoundsAndCosts(ins, g, true, freeCol);
+    return ins;
+}
+
+// P periods of an (bm x bn, perCol per column) box block; `link` linking rows,
+// each touching `linkNnz` random columns of two adjacent periods.
+Instance makeBlock(int periods, int bm, int bn, int perCol, int link, int linkNnz, uint64_t seed) {
+    Instance ins;
+    ins.m = periods * bm + link;
+    ins.n = periods * bn;
+    Rng g(seed);
+    // Linking rows are drawn first so each column's rows can be sorted once.
+    std::vector<std::vector<std::pair<int32_t, double>>> extra(static_cast<size_t>(ins.n));
+    for (int l = 0; l < link; ++l) {
+        const int p = periods > 1 ? static_cast<int>(g.below(static_cast<uint32_t>(periods - 1))) : 0;
+        const int32_t row = periods * bm + l;
+        std::vector<int32_t> cols;
+        while (static_cast<int>(cols.size()) < linkNnz) {
+            const int span = periods > 1 ? 2 * bn : bn;
+            const int32_t col = static_cast<int32_t>(p * bn + static_cast<int>(g.below(static_cast<uint32_t>(span))));
+            if (std::find(cols.begin(), cols.end(), col) == cols.end()) cols.push_back(col);
+        }
+        for (int32_t col : cols) extra[col].push_back({row, nonzeroNormal(g)});
+    }
+    std::vector<int32_t> rows;
+    ins.cp.assign(static_cast<size_t>(ins.n) + 1, 0);
+    for (int p = 0; p < periods; ++p)
+        for (int jj = 0; jj < bn; ++jj) {
+            const int j = p * bn + jj;
+            distinctRows(g, bm, std::min(perCol, bm), rows);
+            std::vector<std::pair<int32_t, double>> entries;
+            for (int32_t r : rows) entries.push_back({p * bm + r, nonzeroNormal(g)});
+            for (const auto& e : extra[j]) entries.push_back(e);
+            std::sort(entries.begin(), entries.end(),
+                      [](const auto& a, const auto& b) { return a.first < b.first; });
+            for (const auto& e : entries) {
+                ins.ri.push_back(e.first);
+                ins.cv.push_back(e.second);
+            }
+            ins.cp[j + 1] = static_cast<int64_t>(ins.ri.size());
+        }
+    finishCsr(ins);
+    ins.x0.resize(static_cast<size_t>(ins.n));
+    for (int j = 0; j < ins.n; ++j) ins.x0[j] = g.uniform(0.0, 10.0);
+    ins.lo.assign(static_cast<size_t>(ins.n), 0.0);
+    ins.hi.assign(static_cast<size_t>(ins.n), 20.0);
+    plantBoundsAndCosts(ins, g, false, std::vector<char>(static_cast<size_t>(ins.n), 0));
+    return ins;
+}
+
+}  // namespace
+
+extern "C" {
+
+// family: 0 = box (param = nnz per column), 1 = zipf (param = 1 for the literal
+// Zipf degrees), 2 = block (rows/cols = block shape, param = periods; the link
+// shape comes from lpgen_block).
+void* lpgen_box(int32_t m, int32_t n, int32_t per_col, uint64_t seed) {
+    return new Instance(makeBox(m, n, per_col, seed));
+}
+void* lpgen_zipf(int32_t m, int32_t n, int32_t literal, uint64_t seed) {
+    return new Instance(makeZipf(m, n, literal != 0, seed));
+}
+void* lpgen_block(int32_t periods, int32_t bm, int32_t bn, int32_t per_col, int32_t link,
+                  int32_t link_nnz, uint64_t seed) {
+    return new Instance(makeBlock(periods, bm, bn, per_col, link, link_nnz, seed));
+}
+
+void lpgen_sizes(const void* h, int32_t* m, int32_t* n, int64_t* nnz) {
+    const Instance& ins = *static_cast<const Instance*>(h);
+    *m = ins.m;
+    *n = ins.n;
+    *nnz = static_cast<int64_t>(ins.cv.size());
+}
+
+void lpgen_export(const void* h, int64_t* rp, int32_t* ci, double* rv, int64_t* cp, int32_t* ri,
+                  double* cv, double* c, double* lo, double* hi, double* rlo, double* rhi,
+                  double* x0) {
+    const Instance& ins = *static_cast<const Instance*>(h);
+    auto put = [](auto* dst, const auto& src) {
+        if (dst) std::memcpy(dst, src.data(), src.size() * sizeof(src[0]));
+    };
+    put(rp, ins.rp);
+    put(ci, ins.ci);
+    put(rv, ins.rv);
+    put(cp, ins.cp);
+    put(ri, ins.ri);
+    put(cv, ins.cv);
+    put(c, ins.c);
+    put(lo, ins.lo);
+    put(hi, ins.hi);
+    put(rlo, ins.rlo);
+    put(rhi, ins.rhi);
+    put(x0, ins.x0);
+}
+
+void lpgen_free(void* h) { delete static_cast<Instance*>(h); }
+
+// Exposed for tests of the deterministic elementary functions.
+double lpgen_log(double x) { return detLog(x); }
+double lpgen_exp(double x) { return detExp(x); }
+
+// Uniform / normal / Zipf streams (for distribution tests).
+void lpgen_stream(uint64_t seed, int32_t kind, int64_t count, double* out) {
+    Rng g(seed);
+    const Zipf zipf(2000);
+    for (int64_t i = 0; i < count; ++i)
+        out[i] = kind == 0 ? g.unit() : (kind == 1 ? g.normal() : static_cast<double>(zipf.draw(g)));
+}
+
+}  // extern "C"

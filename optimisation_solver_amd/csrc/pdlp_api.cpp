@@ -1,0 +1,473 @@
+// C++ drop-in API (include/pdlp/*.h) over the engine's C ABI (include/pdlpx.h).
+//
+// pdlp::PdlpSolver::solve keeps the reference's contract
+// (pdlp_engine/src/pdlp_solver.cpp:116-548 there): the problem and options are
+// borrowed, the result owns its vectors, nothing is thrown for a bad problem
+// or a device error, and each call is independent (one device context and
+// stream per call), so concurrent calls from different host threads are safe.
+
+#include "pdlp/pdlp_solver.h"
+#include "pdlp/scaling.h"
+#include "pdlpx.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <utility>
+
+namespace pdlp {
+
+// ---------------------------------------------------------------------------
+// Executor / SpmvPlan (host utilities of the SparseMatrix interface)
+// ---------------------------------------------------------------------------
+Executor::Executor(int requestedThreads) {
+    const int hw = static_cast<int>(std::thread::hardware_concurrency());
+    threads_ = requestedThreads > 0 ? requestedThreads : std::max(hw, 1);
+}
+
+void Executor::run(const std::function<void(int)>& body) {
+    std::vector<std::thread> pool;
+    pool.reserve(static_cast<size_t>(threads_ - 1));
+    for (int part = 1; part < threads_; ++part) pool.emplace_back([&body, part] { body(part); });
+    body(0);
+    for (auto& t : pool) t.join();
+}
+
+namespace {
+
+// Contiguous slices of [0, count) with about equal nonzeros, cut where the
+// offset array crosses total*k/parts (sparse_matrix.cpp:13-38 there).
+std::vector<int> nnzSplit(const std::vector<SparseMatrix::Offset>& start, int count, int parts) {
+    std::vector<int> cut(static_cast<size_t>(parts) + 1, 0);
+    cut.back() = count;
+    if (parts <= 1 || count <= 0) return cut;
+    const SparseMatrix::Offset total = start[static_cast<size_t>(count)];
+    int from = 0;
+    for (int k = 1; k < parts; ++k) {
+        const SparseMatrix::Offset goal = total * k / parts;
+        const auto it = std::lower_bound(start.begin() + from, start.begin() + count, goal);
+        from = std::clamp(static_cast<int>(it - start.begin()), from, count);
+        cut[static_cast<size_t>(k)] = from;
+    }
+    return cut;
+}
+
+}  // namespace
+
+SpmvPlan SpmvPlan::build(const SparseMatrix& matrix, int parts, int chunksPerPart) {
+    SpmvPlan plan;
+    plan.parts = std::max(parts, 1);
+    const auto nnz = static_cast<SparseMatrix::Offset>(matrix.nonzeros());
+    int chunks = 1;
+    if (plan.parts > 1) {
+        const int wanted = plan.parts * std::max(chunksPerPart, 1);
+        const int affordable = static_cast<int>(std::max<SparseMatrix::Offset>(nnz / 4096, 1));
+        chunks = std::clamp(std::min(wanted, affordable), plan.parts, wanted);
+    }
+    plan.rowChunk = nnzSplit(matrix.csrRowStart(), matrix.rows(), chunks);
+    plan.columnChunk = nnzSplit(matrix.cscColumnStart(), matrix.columns(), chunks);
+    return plan;
+}
+
+// ---------------------------------------------------------------------------
+// SparseMatrix (host side; sparse_matrix.cpp:76-419 there)
+// ---------------------------------------------------------------------------
+SparseMatrix SparseMatrix::fromTriplets(int rows, int columns, std::vector<MatrixTriplet> triplets) {
+    if (rows < 0 || columns < 0) throw std::invalid_argument("Sparse matrix dimensions must be nonnegative");
+    for (const MatrixTriplet& t : triplets) {
+        if (t.row < 0 || t.row >= rows || t.column < 0 || t.column >= columns)
+            throw std::invalid_argument("Sparse matrix triplet index is out of range");
+        if (!std::isfinite(t.value)) throw std::invalid_argument("Sparse matrix coefficient must be finite");
+    }
+    SparseMatrix a;
+    a.rows_ = rows;
+    a.columns_ = columns;
+    // bucket by row
+    std::vector<Offset> head(static_cast<size_t>(rows) + 1, 0);
+    for (const MatrixTriplet& t : triplets) ++head[static_cast<size_t>(t.row) + 1];
+    for (int r = 0; r < rows; ++r) head[r + 1] += head[r];
+    std::vector<std::pair<int, double>> bucket(triplets.size());
+    {
+        std::vector<Offset> pos(head.begin(), head.end() - 1);
+        for (const MatrixTriplet& t : triplets) bucket[static_cast<size_t>(pos[t.row]++)] = {t.column, t.value};
+    }
+    triplets.clear();
+    triplets.shrink_to_fit();
+    // sort each row by column, merge duplicates, drop exact zeros
+    a.csrRowStart_.assign(static_cast<size_t>(rows) + 1, 0);
+    a.csrColumnIndex_.reserve(bucket.size());
+    a.csrValues_.reserve(bucket.size());
+    for (int r = 0; r < rows; ++r) {
+        auto first = bucket.begin() + head[r];
+        auto last = bucket.begin() + head[r + 1];
+        std::sort(first, last, [](const auto& l, const auto& q) { return l.first < q.first; });
+        for (auto it = first; it != last;) {
+            const int col = it->first;
+            double sum = it->second;
+            for (++it; it != last && it->first == col; ++it) sum += it->second;
+            if (sum != 0.0) {
+                a.csrColumnIndex_.push_back(col);
+                a.csrValues_.push_back(sum);
+            }
+        }
+        a.csrRowStart_[static_cast<size_t>(r) + 1] = static_cast<Offset>(a.csrValues_.size());
+    }
+    a.csrColumnIndex_.shrink_to_fit();
+    a.csrValues_.shrink_to_fit();
+    // CSC by a row-order scan: columns come out sorted by row
+    const size_t nnz = a.csrValues_.size();
+    a.cscColumnStart_.assign(static_cast<size_t>(columns) + 1, 0);
+    for (int c : a.csrColumnIndex_) ++a.cscColumnStart_[static_cast<size_t>(c) + 1];
+    for (int c = 0; c < columns; ++c) a.cscColumnStart_[c + 1] += a.cscColumnStart_[c];
+    a.cscRowIndex_.resize(nnz);
+    a.cscValues_.resize(nnz);
+    std::vector<Offset> pos(a.cscColumnStart_.begin(), a.cscColumnStart_.end() - 1);
+    for (int r = 0; r < rows; ++r)
+        for (Offset k = a.csrRowStart_[r]; k < a.csrRowStart_[r + 1]; ++k) {
+            const Offset at = pos[static_cast<size_t>(a.csrColumnIndex_[k])]++;
+            a.cscRowIndex_[static_cast<size_t>(at)] = r;
+            a.cscValues_[static_cast<size_t>(at)] = a.csrValues_[static_cast<size_t>(k)];
+        }
+    return a;
+}
+
+bool SparseMatrix::validate() const noexcept {
+    const auto nnz = static_cast<Offset>(csrValues_.size());
+    if (rows_ < 0 || columns_ < 0) return false;
+    if (csrRowStart_.size() != static_cast<size_t>(rows_) + 1 ||
+        cscColumnStart_.size() != static_cast<size_t>(columns_) + 1)
+        return false;
+    if (csrColumnIndex_.size() != csrValues_.size() || cscRowIndex_.size() != cscValues_.size() ||
+        csrValues_.size() != cscValues_.size())
+        return false;
+    if (csrRowStart_.front() != 0 || cscColumnStart_.front() != 0 || csrRowStart_.back() != nnz ||
+        cscColumnStart_.back() != nnz)
+        return false;
+    if (!std::is_sorted(csrRowStart_.begin(), csrRowStart_.end())) return false;
+    if (!std::is_sorted(cscColumnStart_.begin(), cscColumnStart_.end())) return false;
+    for (size_t k = 0; k < csrValues_.size(); ++k)
+        if (csrColumnIndex_[k] < 0 || csrColumnIndex_[k] >= columns_ || !std::isfinite(csrValues_[k])) return false;
+    for (size_t k = 0; k < cscValues_.size(); ++k)
+        if (cscRowIndex_[k] < 0 || cscRowIndex_[k] >= rows_ || !std::isfinite(cscValues_[k])) return false;
+    return true;
+}
+
+SparseMatrix SparseMatrix::scaled(const std::vector<double>& rowScale, const std::vector<double>& columnScale) const {
+    if (rowScale.size() != static_cast<size_t>(rows_) || columnScale.size() != static_cast<size_t>(columns_))
+        throw std::invalid_argument("Scaling vectors do not match the matrix dimensions");
+    SparseMatrix out = *this;
+    for (int r = 0; r < rows_; ++r)
+        for (Offset k = csrRowStart_[r]; k < csrRowStart_[r + 1]; ++k)
+            out.csrValues_[static_cast<size_t>(k)] *= rowScale[r] * columnScale[csrColumnIndex_[k]];
+    for (int c = 0; c < columns_; ++c)
+        for (Offset k = cscColumnStart_[c]; k < cscColumnStart_[c + 1]; ++k)
+            out.cscValues_[static_cast<size_t>(k)] *= columnScale[c] * rowScale[cscRowIndex_[k]];
+    return out;
+}
+
+namespace {
+
+inline double quadDot(SparseMatrix::Offset b, SparseMatrix::Offset e, const int* idx, const double* val,
+                      const double* v) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    SparseMatrix::Offset k = b;
+    for (; k + 4 <= e; k += 4) {
+        s0 += val[k] * v[idx[k]];
+        s1 += val[k + 1] * v[idx[k + 1]];
+        s2 += val[k + 2] * v[idx[k + 2]];
+        s3 += val[k + 3] * v[idx[k + 3]];
+    }
+    for (; k < e; ++k) s0 += val[k] * v[idx[k]];
+    return (s0 + s1) + (s2 + s3);
+}
+
+template <typename Slice>
+void chunked(Executor* ex, const SpmvPlan* plan, const std::vector<int>& chunk, int total, Slice&& slice) {
+    if (!ex || !plan || plan->parts <= 1) {
+        slice(0, total);
+        return;
+    }
+    const int count = static_cast<int>(chunk.size()) - 1;
+    std::atomic<int> next{0};
+    ex->run([&](int) {
+        for (int q = next.fetch_add(1); q < count; q = next.fetch_add(1)) slice(chunk[q], chunk[q + 1]);
+    });
+}
+
+}  // namespace
+
+void SparseMatrix::multiplyRange(const double* x, double* result, int rowBegin, int rowEnd) const noexcept {
+    for (int r = rowBegin; r < rowEnd; ++r)
+        result[r] = quadDot(csrRowStart_[r], csrRowStart_[r + 1], csrColumnIndex_.data(), csrValues_.data(), x);
+}
+
+void SparseMatrix::transposeMultiplyRange(const double* y, double* result, int columnBegin,
+                                          int columnEnd) const noexcept {
+    for (int c = columnBegin; c < columnEnd; ++c)
+        result[c] = quadDot(cscColumnStart_[c], cscColumnStart_[c + 1], cscRowIndex_.data(), cscValues_.data(), y);
+}
+
+void SparseMatrix::multiply(const std::vector<double>& x, std::vector<double>& result, Executor* executor,
+                            const SpmvPlan* plan) const {
+    if (x.size() != static_cast<size_t>(columns_))
+        throw std::invalid_argument("A*x received a vector with the wrong dimension");
+    result.resize(static_cast<size_t>(rows_));
+    const double* in = x.data();
+    double* out = result.data();
+    chunked(executor, plan, plan ? plan->rowChunk : std::vector<int>{}, rows_,
+            [&](int b, int e) { multiplyRange(in, out, b, e); });
+}
+
+void SparseMatrix::transposeMultiply(const std::vector<double>& y, std::vector<double>& result,
+                                     Executor* executor, const SpmvPlan* plan) const {
+    if (y.size() != static_cast<size_t>(rows_))
+        throw std::invalid_argument("A^T*y received a vector with the wrong dimension");
+    result.resize(static_cast<size_t>(columns_));
+    const double* in = y.data();
+    double* out = result.data();
+    chunked(executor, plan, plan ? plan->columnChunk : std::vector<int>{}, columns_,
+            [&](int b, int e) { transposeMultiplyRange(in, out, b, e); });
+}
+
+// ---------------------------------------------------------------------------
+// Problem view for the C ABI
+// ---------------------------------------------------------------------------
+namespace {
+
+pdlpx_problem view(const CompiledLp& lp) {
+    pdlpx_problem p{};
+    const SparseMatrix& a = lp.matrix;
+    p.rows = a.rows();
+    p.cols = a.columns();
+    p.nnz = static_cast<int64_t>(a.nonzeros());
+    p.csr_row_start = a.csrRowStart().data();
+    p.csr_col = a.csrColumnIndex().data();
+    p.csr_val = a.csrValues().data();
+    p.csc_col_start = a.cscColumnStart().data();
+    p.csc_row = a.cscRowIndex().data();
+    p.csc_val = a.cscValues().data();
+    p.objective = lp.objective.data();
+    p.objective_offset = lp.objectiveOffset;
+    p.row_lower = lp.rowLower.data();
+    p.row_upper = lp.rowUpper.data();
+    p.var_lower = lp.variableLower.data();
+    p.var_upper = lp.variableUpper.data();
+    return p;
+}
+
+pdlpx_options convert(const PdlpOptions& o) {
+    pdlpx_options r;
+    pdlpx_default_options(&r);
+    r.iteration_limit = o.iterationLimit;
+    r.time_limit_seconds = o.timeLimitSeconds;
+    r.primal_tolerance = o.primalTolerance;
+    r.dual_tolerance = o.dualTolerance;
+    r.gap_tolerance = o.gapTolerance;
+    r.termination_check_frequency = o.terminationCheckFrequency;
+    r.power_iteration_count = o.powerIterationCount;
+    r.thread_count = o.threadCount;
+    r.parallel_nonzero_threshold = o.parallelNonzeroThreshold;
+    r.initial_step_safety = o.initialStepSafety;
+    r.minimum_step_scale = o.minimumStepScale;
+    r.maximum_primal_weight = o.maximumPrimalWeight;
+    r.minimum_primal_weight = o.minimumPrimalWeight;
+    r.use_ruiz_scaling = o.useRuizScaling;
+    r.ruiz_iterations = o.ruizIterations;
+    r.detect_infeasibility = o.detectInfeasibility;
+    r.infeasibility_tolerance = o.infeasibilityTolerance;
+    r.use_diagonal_preconditioning = o.useDiagonalPreconditioning;
+    r.use_adaptive_linesearch = o.useAdaptiveLinesearch;
+    r.maximum_step_trials = o.maximumStepTrials;
+    r.use_averaging = o.useAveraging;
+    r.use_restarts = o.useRestarts;
+    r.use_feasibility_polishing = o.useFeasibilityPolishing;
+    r.minimum_restart_iterations = o.minimumRestartIterations;
+    r.maximum_restart_iterations = o.maximumRestartIterations;
+    r.restart_sufficient_decay = o.restartSufficientDecay;
+    r.restart_necessary_decay = o.restartNecessaryDecay;
+    r.restart_artificial_fraction = o.restartArtificialFraction;
+    r.initial_primal_weight = o.initialPrimalWeight;
+    r.primal_weight_smoothing = o.primalWeightSmoothing;
+    r.polishing_iterations = o.polishingIterations;
+    r.precision = o.precision == Precision::Fp32 ? PDLPX_FP32 : PDLPX_FP64;
+    r.fp32_handoff_factor = o.fp32HandoffFactor;
+    r.fp32_stall_checks = o.fp32StallChecks;
+    return r;
+}
+
+struct Context {
+    pdlpx_ctx* ctx = nullptr;
+    int rc = 0;
+    explicit Context(int device) { rc = pdlpx_create(&ctx, device); }
+    ~Context() { pdlpx_destroy(ctx); }
+    std::string error() const {
+        return ctx ? std::string(pdlpx_last_error(ctx)) : "HIP: device context could not be created";
+    }
+};
+
+PdlpResult failure(PdlpStatus status, std::string message, double seconds) {
+    PdlpResult r;
+    r.status = status;
+    r.statusMessage = std::move(message);
+    r.solveTimeSeconds = seconds;
+    return r;
+}
+
+PdlpResult runSolve(const CompiledLp& problem, const PdlpOptions& options, const WarmStart* warm) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const auto seconds = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    try {
+        problem.validate();
+    } catch (const std::exception& e) {
+        return failure(PdlpStatus::InvalidProblem, e.what(), seconds());
+    }
+    const int n = problem.numColumns(), m = problem.numRows();
+    if (warm && ((!warm->primal.empty() && warm->primal.size() != static_cast<size_t>(n)) ||
+                 (!warm->rowDual.empty() && warm->rowDual.size() != static_cast<size_t>(m))))
+        return failure(PdlpStatus::InvalidProblem, "Warm start dimensions do not match the LP", seconds());
+    Context c(options.device);
+    if (c.rc != PDLPX_OK) return failure(PdlpStatus::NumericalFailure, c.error(), seconds());
+    const pdlpx_problem p = view(problem);
+    const pdlpx_options o = convert(options);
+    int rc = pdlpx_load(c.ctx, &p, o.precision);
+    if (rc != PDLPX_OK)
+        return failure(rc == PDLPX_E_NOMEM ? PdlpStatus::InvalidProblem : PdlpStatus::NumericalFailure, c.error(),
+                       seconds());
+    PdlpResult r;
+    r.primal.resize(static_cast<size_t>(n));
+    r.rowDual.resize(static_cast<size_t>(m));
+    std::vector<double> ray(static_cast<size_t>(std::max(n, m)));
+    pdlpx_summary s{};
+    const double* wx = warm && !warm->primal.empty() ? warm->primal.data() : nullptr;
+    const double* wy = warm && !warm->rowDual.empty() ? warm->rowDual.data() : nullptr;
+    rc = pdlpx_solve(c.ctx, &o, wx, wy, &s, r.primal.data(), r.rowDual.data(), ray.data());
+    if (rc != PDLPX_OK)
+        return failure(rc == PDLPX_E_NOMEM ? PdlpStatus::InvalidProblem : PdlpStatus::NumericalFailure, c.error(),
+                       seconds());
+    r.status = static_cast<PdlpStatus>(s.status);
+    r.statusMessage = s.message;
+    if (r.status == PdlpStatus::InvalidProblem) {
+        r.primal.clear();
+        r.rowDual.clear();
+    }
+    if (s.has_dual_ray) r.dualRay.assign(ray.begin(), ray.begin() + m);
+    if (s.has_primal_ray) r.primalRay.assign(ray.begin(), ray.begin() + n);
+    r.primalObjective = s.primal_objective;
+    r.dualObjective = s.dual_objective;
+    r.primalResidual = s.primal_residual;
+    r.dualResidual = s.dual_residual;
+    r.relativeGap = s.relative_gap;
+    r.iterations = s.iterations;
+    r.stepTrials = s.step_trials;
+    r.restartCount = s.restart_count;
+    r.finalStepSize = s.final_step_size;
+    r.staticStepBound = s.static_step_bound;
+    r.finalPrimalWeight = s.final_primal_weight;
+    r.solveTimeSeconds = seconds();
+    return r;
+}
+
+}  // namespace
+
+const char* toString(PdlpStatus status) noexcept {
+    switch (status) {
+        case PdlpStatus::Optimal: return "optimal";
+        case PdlpStatus::Infeasible: return "infeasible";
+        case PdlpStatus::Unbounded: return "unbounded";
+        case PdlpStatus::IterationLimit: return "iteration_limit";
+        case PdlpStatus::TimeLimit: return "time_limit";
+        case PdlpStatus::NumericalFailure: return "numerical_failure";
+        case PdlpStatus::InvalidProblem: return "invalid_problem";
+    }
+    return "unknown";
+}
+
+PdlpResult PdlpSolver::solve(const CompiledLp& problem, const PdlpOptions& options) const {
+    return runSolve(problem, options, nullptr);
+}
+
+PdlpResult PdlpSolver::solve(const CompiledLp& problem, const PdlpOptions& options,
+                             const WarmStart& warmStart) const {
+    return runSolve(problem, options, &warmStart);
+}
+
+// ---------------------------------------------------------------------------
+// Ruiz scaling through the device set-up
+// ---------------------------------------------------------------------------
+void ProblemScaling::toOriginal(const std::vector<double>& scaledPrimal, const std::vector<double>& scaledDual,
+                                std::vector<double>& primal, std::vector<double>& dual) const {
+    primal.resize(scaledPrimal.size());
+    for (size_t j = 0; j < scaledPrimal.size(); ++j) primal[j] = columnScale[j] * scaledPrimal[j];
+    dual.resize(scaledDual.size());
+    for (size_t i = 0; i < scaledDual.size(); ++i) dual[i] = rowScale[i] * scaledDual[i];
+}
+
+void ProblemScaling::toScaled(const std::vector<double>& primal, const std::vector<double>& dual,
+                              std::vector<double>& scaledPrimal, std::vector<double>& scaledDual) const {
+    scaledPrimal.resize(primal.size());
+    for (size_t j = 0; j < primal.size(); ++j) scaledPrimal[j] = primal[j] / columnScale[j];
+    scaledDual.resize(dual.size());
+    for (size_t i = 0; i < dual.size(); ++i) scaledDual[i] = dual[i] / rowScale[i];
+}
+
+ProblemScaling RuizScaler::equilibrate(const CompiledLp& problem, int iterations) {
+    ProblemScaling out;
+    const int m = problem.numRows(), n = problem.numColumns();
+    out.rowScale.assign(static_cast<size_t>(m), 1.0);
+    out.columnScale.assign(static_cast<size_t>(n), 1.0);
+    if (iterations > 0 && problem.matrix.nonzeros() > 0) {
+        Context c(0);
+        if (c.rc != PDLPX_OK) throw std::runtime_error(c.error());
+        const pdlpx_problem p = view(problem);
+        if (pdlpx_load(c.ctx, &p, PDLPX_FP64) != PDLPX_OK ||
+            pdlpx_scaling(c.ctx, iterations, 0, out.rowScale.data(), out.columnScale.data(), nullptr, nullptr) !=
+                PDLPX_OK)
+            throw std::runtime_error(c.error());
+    }
+    CompiledLp& s = out.problem;
+    s.matrix = problem.matrix.scaled(out.rowScale, out.columnScale);
+    s.objectiveOffset = problem.objectiveOffset;
+    s.objective.resize(static_cast<size_t>(n));
+    s.variableLower.resize(static_cast<size_t>(n));
+    s.variableUpper.resize(static_cast<size_t>(n));
+    for (int j = 0; j < n; ++j) {
+        const double f = out.columnScale[j];
+        s.objective[j] = problem.objective[j] * f;
+        s.variableLower[j] = std::isfinite(problem.variableLower[j]) ? problem.variableLower[j] * (1.0 / f)
+                                                                     : problem.variableLower[j];
+        s.variableUpper[j] = std::isfinite(problem.variableUpper[j]) ? problem.variableUpper[j] * (1.0 / f)
+                                                                     : problem.variableUpper[j];
+    }
+    s.rowLower.resize(static_cast<size_t>(m));
+    s.rowUpper.resize(static_cast<size_t>(m));
+    for (int i = 0; i < m; ++i) {
+        const double f = out.rowScale[i];
+        s.rowLower[i] = std::isfinite(problem.rowLower[i]) ? problem.rowLower[i] * f : problem.rowLower[i];
+        s.rowUpper[i] = std::isfinite(problem.rowUpper[i]) ? problem.rowUpper[i] * f : problem.rowUpper[i];
+    }
+    return out;
+}
+
+double RuizScaler::conditionSpread(const SparseMatrix& matrix) {
+    double lo = std::numeric_limits<double>::infinity(), hi = 0.0;
+    const auto& start = matrix.csrRowStart();
+    const auto& val = matrix.csrValues();
+    for (int r = 0; r < matrix.rows(); ++r) {
+        double peak = 0.0;
+        for (auto k = start[r]; k < start[r + 1]; ++k) peak = std::max(peak, std::abs(val[static_cast<size_t>(k)]));
+        if (peak > 0.0) {
+            lo = std::min(lo, peak);
+            hi = std::max(hi, peak);
+        }
+    }
+    if (!std::isfinite(lo) || lo <= 0.0) return 1.0;
+    return hi / lo;
+}
+
+}  // namespace pdlp

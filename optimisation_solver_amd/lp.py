@@ -1,0 +1,173 @@
+"""LP container in the engine's storage layout and the seeded instance families.
+
+``Lp`` holds exactly what ``pdlpx_problem`` (include/pdlpx.h) borrows: CSR
+(int64 offsets, int32 columns, float64 values), the CSC copy with columns
+sorted by row, and the bound / cost vectors, with IEEE infinities for absent
+bounds -- the layout of the reference's CompiledLp
+(pdlp_engine/include/pdlp/compiled_lp.h:17-63, sparse_matrix.h:52-135).
+
+Benchmark families (csrc/lpgen.cpp): base seed 20261203; configuration k uses
+seed base + k, instance i of a batch uses base + 1000 k + i.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+BASE_SEED = 20261203
+INF = math.inf
+
+
+@dataclass
+class Lp:
+    m: int
+    n: int
+    rp: np.ndarray   # int64 [m+1]
+    ci: np.ndarray   # int32 [nnz]
+    rv: np.ndarray   # float64 [nnz]
+    cp: np.ndarray   # int64 [n+1]
+    ri: np.ndarray   # int32 [nnz]
+    cv: np.ndarray   # float64 [nnz]
+    c: np.ndarray
+    lo: np.ndarray
+    hi: np.ndarray
+    rlo: np.ndarray
+    rhi: np.ndarray
+    offset: float = 0.0
+    x0: np.ndarray | None = None   # planted feasible point, when generated
+    name: str = ""
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.rv.size)
+
+    # -- construction --------------------------------------------------------
+    @staticmethod
+    def from_triplets(m: int, n: int, rows, cols, vals, c, lo, hi, rlo, rhi, offset: float = 0.0,
+                      name: str = "") -> "Lp":
+        """Canonical CSR/CSC from triplets: sorted, duplicates summed (in input
+        order), exact zeros dropped -- SparseMatrix::fromTriplets semantics
+        (pdlp_engine/src/sparse_matrix.cpp:76-201)."""
+        rows = np.asarray(rows, dtype=np.int64)
+        cols = np.asarray(cols, dtype=np.int64)
+        vals = np.asarray(vals, dtype=np.float64)
+        if rows.size and (rows.min() < 0 or rows.max() >= m or cols.min() < 0 or cols.max() >= n):
+            raise ValueError("Sparse matrix triplet index is out of range")
+        if not np.all(np.isfinite(vals)):
+            raise ValueError("Sparse matrix coefficient must be finite")
+        key = rows * max(n, 1) + cols
+        order = np.argsort(key, kind="stable")
+        key, vals_s = key[order], vals[order]
+        uniq, start = np.unique(key, return_index=True)
+        summed = np.empty(uniq.size)
+        for q, (a, b) in enumerate(zip(start, list(start[1:]) + [key.size])):
+            s = 0.0
+            for v in vals_s[a:b]:
+                s += v
+            summed[q] = s
+        keep = summed != 0.0
+        uniq, summed = uniq[keep], summed[keep]
+        r = (uniq // max(n, 1)).astype(np.int64)
+        cc = (uniq % max(n, 1)).astype(np.int64)
+        return Lp.from_coo_sorted(m, n, r, cc, summed, c, lo, hi, rlo, rhi, offset, name)
+
+    @staticmethod
+    def from_coo_sorted(m, n, r, cc, v, c, lo, hi, rlo, rhi, offset=0.0, name="") -> "Lp":
+        rp = np.zeros(m + 1, dtype=np.int64)
+        np.add.at(rp, r + 1, 1)
+        rp = np.cumsum(rp).astype(np.int64)
+        corder = np.lexsort((r, cc))   # by column, then row
+        cp = np.zeros(n + 1, dtype=np.int64)
+        np.add.at(cp, cc + 1, 1)
+        cp = np.cumsum(cp).astype(np.int64)
+        f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        return Lp(m, n, rp, cc.astype(np.int32), f(v), cp, r[corder].astype(np.int32), f(np.asarray(v)[corder]),
+                  f(c), f(lo), f(hi), f(rlo), f(rhi), float(offset), name=name)
+
+    # -- C view --------------------------------------------------------------
+    def problem(self) -> _native.Problem:
+        p = _native.Problem()
+        p.rows, p.cols, p.nnz = self.m, self.n, self.nnz
+        arrs = [self.rp, self.ci, self.rv, self.cp, self.ri, self.cv, self.c, self.rlo, self.rhi,
+                self.lo, self.hi]
+        for a in arrs:
+            assert a.flags["C_CONTIGUOUS"]
+        p.csr_row_start, p.csr_col, p.csr_val = (a.ctypes.data for a in arrs[:3])
+        p.csc_col_start, p.csc_row, p.csc_val = (a.ctypes.data for a in arrs[3:6])
+        p.objective = self.c.ctypes.data
+        p.objective_offset = self.offset
+        p.row_lower, p.row_upper = self.rlo.ctypes.data, self.rhi.ctypes.data
+        p.var_lower, p.var_upper = self.lo.ctypes.data, self.hi.ctypes.data
+        return p
+
+    def sha256(self) -> str:
+        h = hashlib.sha256()
+        for a in (self.rp, self.ci, self.rv, self.cp, self.ri, self.cv, self.c, self.lo, self.hi,
+                  self.rlo, self.rhi):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()
+
+    def with_costs_and_rows(self, c, rlo, rhi, name: str = "") -> "Lp":
+        """Same matrix and column bounds, new costs and row bounds."""
+        f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        return Lp(self.m, self.n, self.rp, self.ci, self.rv, self.cp, self.ri, self.cv, f(c), self.lo,
+                  self.hi, f(rlo), f(rhi), self.offset, x0=self.x0, name=name or self.name)
+
+
+def _export(handle, name: str) -> Lp:
+    lib = _native.lpgen()
+    m, n, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    lib.lpgen_sizes(handle, C.byref(m), C.byref(n), C.byref(nnz))
+    m, n, nnz = m.value, n.value, nnz.value
+    rp = np.empty(m + 1, np.int64)
+    ci = np.empty(nnz, np.int32)
+    rv = np.empty(nnz)
+    cp = np.empty(n + 1, np.int64)
+    ri = np.empty(nnz, np.int32)
+    cv = np.empty(nnz)
+    c, lo, hi, x0 = (np.empty(n) for _ in range(4))
+    rlo, rhi = np.empty(m), np.empty(m)
+    lib.lpgen_export(handle, *(a.ctypes.data for a in (rp, ci, rv, cp, ri, cv, c, lo, hi, rlo, rhi, x0)))
+    lib.lpgen_free(handle)
+    return Lp(m, n, rp, ci, rv, cp, ri, cv, c, lo, hi, rlo, rhi, 0.0, x0=x0, name=name)
+
+
+def box_lp(m: int, n: int, per_col: int = 5, seed: int = BASE_SEED) -> Lp:
+    """Uniform-row family with box bounds [0, 20] (the CPU-runnable config)."""
+    return _export(_native.lpgen().lpgen_box(m, n, per_col, seed), f"box_{m}x{n}_s{seed}")
+
+
+def zipf_lp(m: int, n: int, literal: bool = False, seed: int = BASE_SEED + 1) -> Lp:
+    """Power-law column degrees, lognormal row scaling, 10% free columns."""
+    return _export(_native.lpgen().lpgen_zipf(m, n, int(literal), seed), f"zipf_{m}x{n}_s{seed}")
+
+
+def block_lp(periods: int = 24, bm: int = 2000, bn: int = 5000, per_col: int = 5, link: int = 4000,
+             link_nnz: int = 12, seed: int = BASE_SEED + 4) -> Lp:
+    """Multi-period block-angular family (period blocks + linking rows)."""
+    h = _native.lpgen().lpgen_block(periods, bm, bn, per_col, link, link_nnz, seed)
+    return _export(h, f"block_{periods}x{bm}x{bn}_s{seed}")
+
+
+def batch_sizes(count: int = 256, seed: int = BASE_SEED + 3000) -> list[tuple[int, int]]:
+    """(m, n) of the batched family: m log-uniform in [500, 20000], n = 2.5 m."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u = rng.random(count)
+    ms = np.rint(np.exp(np.log(500.0) + u * (np.log(20000.0) - np.log(500.0)))).astype(int)
+    return [(int(mm), int(round(2.5 * mm))) for mm in ms]
+
+
+# Named workloads (sizes as the benchmark configurations give them).
+CONFIGS = {
+    0: dict(name="c0_box_2000x5000", make=lambda: box_lp(2000, 5000, 5, BASE_SEED + 0), tol=1e-4),
+    1: dict(name="c1_zipf_200000x500000", make=lambda: zipf_lp(200_000, 500_000, False, BASE_SEED + 1), tol=1e-6),
+    2: dict(name="c2_zipf_1000000x2000000", make=lambda: zipf_lp(1_000_000, 2_000_000, False, BASE_SEED + 2),
+            tol=1e-6),
+}

@@ -1,0 +1,156 @@
+"""Python mirror of the drop-in solve interface over the engine's C ABI.
+
+``PdlpSolver().solve(lp, options)`` follows pdlp::PdlpSolver::solve
+(pdlp_engine/src/pdlp_solver.cpp:116-548): same option names (snake_case),
+same statuses and messages, never raises for a bad LP (InvalidProblem), and
+every call runs on the GPU through libpdlp_amd.so.  ``Engine`` keeps one
+device context with an LP resident in HBM for repeated solves (benchmarks).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+from . import _native
+from .lp import Lp
+
+Options = _native.Options
+
+
+def default_options(**overrides) -> _native.Options:
+    """The reference's PdlpOptions defaults (pdlp_options.h:7-75), overridable."""
+    o = _native.Options()
+    _native.engine().pdlpx_default_options(C.byref(o))
+    names = {f[0] for f in _native.Options._fields_}
+    for k, v in overrides.items():
+        if k not in names:
+            raise KeyError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class PdlpResult:
+    status: str
+    status_message: str
+    primal: np.ndarray
+    row_dual: np.ndarray
+    dual_ray: np.ndarray | None
+    primal_ray: np.ndarray | None
+    primal_objective: float
+    dual_objective: float
+    primal_residual: float
+    dual_residual: float
+    relative_gap: float
+    iterations: int
+    step_trials: int
+    restart_count: int
+    solve_time_seconds: float
+    final_step_size: float
+    static_step_bound: float
+    final_primal_weight: float
+    diagnostics: dict = field(default_factory=dict)
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class Engine:
+    """One HIP device context with one LP loaded (kept resident in HBM)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _native.engine()
+        self.ctx = C.c_void_p()
+        rc = self.lib.pdlpx_create(C.byref(self.ctx), device)
+        if rc != 0:
+            raise EngineError(f"pdlpx_create failed ({rc})")
+        self.lp: Lp | None = None
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise EngineError(f"{what} failed ({rc}): {self.lib.pdlpx_last_error(self.ctx).decode()}")
+
+    def load(self, lp: Lp, precision: int = _native.FP64) -> None:
+        self._prob = lp.problem()
+        self._check(self.lib.pdlpx_load(self.ctx, C.byref(self._prob), precision), "pdlpx_load")
+        self.lp = lp
+
+    def solve(self, options: _native.Options | None = None, warm_x=None, warm_y=None) -> PdlpResult:
+        assert self.lp is not None, "load an LP first"
+        lp = self.lp
+        o = options if options is not None else default_options()
+        x = np.zeros(lp.n)
+        y = np.zeros(lp.m)
+        ray = np.zeros(max(lp.n, lp.m, 1))
+        s = _native.Summary()
+        wx = None if warm_x is None else np.ascontiguousarray(warm_x, dtype=np.float64)
+        wy = None if warm_y is None else np.ascontiguousarray(warm_y, dtype=np.float64)
+        rc = self.lib.pdlpx_solve(self.ctx, C.byref(o), None if wx is None else wx.ctypes.data,
+                                  None if wy is None else wy.ctypes.data, C.byref(s), x.ctypes.data,
+                                  y.ctypes.data, ray.ctypes.data)
+        self._check(rc, "pdlpx_solve")
+        return _result(s, x, y, ray, lp)
+
+    def spmv(self, v: np.ndarray, transpose: bool = False) -> np.ndarray:
+        lp = self.lp
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(lp.n if transpose else lp.m)
+        self._check(self.lib.pdlpx_spmv(self.ctx, int(transpose), v.ctypes.data, out.ctypes.data), "pdlpx_spmv")
+        return out
+
+    def scaling(self, ruiz_iterations: int = 10, preconditioning: bool = True):
+        lp = self.lp
+        rs, cs, tau, sig = np.zeros(lp.m), np.zeros(lp.n), np.zeros(lp.n), np.zeros(lp.m)
+        self._check(self.lib.pdlpx_scaling(self.ctx, ruiz_iterations, int(preconditioning), rs.ctypes.data,
+                                           cs.ctypes.data, tau.ctypes.data, sig.ctypes.data), "pdlpx_scaling")
+        return rs, cs, tau, sig
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.pdlpx_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _result(s: _native.Summary, x, y, ray, lp: Lp) -> PdlpResult:
+    d = s.as_dict()
+    invalid = s.status == 6
+    return PdlpResult(
+        status=d["status_name"], status_message=d["message"],
+        primal=np.zeros(0) if invalid else x, row_dual=np.zeros(0) if invalid else y,
+        dual_ray=ray[: lp.m].copy() if s.has_dual_ray else None,
+        primal_ray=ray[: lp.n].copy() if s.has_primal_ray else None,
+        primal_objective=s.primal_objective, dual_objective=s.dual_objective,
+        primal_residual=s.primal_residual, dual_residual=s.dual_residual, relative_gap=s.relative_gap,
+        iterations=s.iterations, step_trials=s.step_trials, restart_count=s.restart_count,
+        solve_time_seconds=s.solve_time_seconds, final_step_size=s.final_step_size,
+        static_step_bound=s.static_step_bound, final_primal_weight=s.final_primal_weight,
+        diagnostics={k: d[k] for k in ("fp32_iterations", "device_loop_seconds", "setup_seconds",
+                                         "termination_checks")})
+
+
+class PdlpSolver:
+    """Stateless facade: each solve gets its own device context."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+
+    def solve(self, lp: Lp, options: _native.Options | None = None, warm_x=None, warm_y=None) -> PdlpResult:
+        o = options if options is not None else default_options()
+        with Engine(self.device) as e:
+            e.load(lp, o.precision)
+            return e.solve(o, warm_x, warm_y)

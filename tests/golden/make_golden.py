@@ -1,0 +1,84 @@
+"""Writes the golden fixtures of the PDLP path from the reference compiled from
+its own sources (oracle/_ref/libref_pdlp.so, built by `make -C oracle ref`).
+
+Fixtures (data only):
+  c0_ref.npz            C0 instance hash, reference x / y, summary scalars
+  test_problems_ref.json reference summaries of the closed-form test LPs and
+                        the seeded random-feasible instances
+  instance_hashes.json  SHA-256 of the generated benchmark instances
+
+Run from the repository root: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "tests"))
+
+import oracle_lib as ol  # noqa: E402
+import problems  # noqa: E402
+from optimisation_solver_amd import box_lp, default_options, zipf_lp  # noqa: E402
+
+OUT = Path(__file__).resolve().parent
+KEYS = ["status_name", "message", "iterations", "step_trials", "restart_count", "primal_objective",
+        "dual_objective", "primal_residual", "dual_residual", "relative_gap", "final_step_size",
+        "static_step_bound", "final_primal_weight"]
+
+
+def opts(tol, **kw):
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, thread_count=1)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def cases():
+    strict = dict(iteration_limit=100000, termination_check_frequency=50, polishing_iterations=2000)
+    out = []
+    for name, (lp, *_rest) in problems.closed_form().items():
+        tol = 1e-8 if name == "ill_conditioned" else 2e-6
+        out.append((name, lp, opts(tol, **strict)))
+    lim = dict(strict, iteration_limit=20000)
+    out.append(("infeasible_rows", problems.infeasible_rows(), opts(2e-6, **lim)))
+    out.append(("infeasible_equalities", problems.infeasible_equalities(), opts(2e-6, **lim)))
+    out.append(("unbounded", problems.unbounded(), opts(2e-6, **lim)))
+    out.append(("unbounded_no_ray_rows", problems.unbounded_no_ray_rows(), opts(2e-6, **dict(strict, iteration_limit=2000))))
+    conv = dict(iteration_limit=60000, termination_check_frequency=40, use_feasibility_polishing=0)
+    out.append(("random_feasible_21", problems.random_feasible(300, 420, 21), opts(1e-8, **conv)))
+    out.append(("random_feasible_33_fixed", problems.random_feasible(260, 360, 33),
+                opts(1e-8, **dict(conv, use_adaptive_linesearch=0))))
+    out.append(("random_feasible_33_nopc", problems.random_feasible(260, 360, 33),
+                opts(1e-6, **dict(conv, use_diagonal_preconditioning=0))))
+    out.append(("random_feasible_polish", problems.random_feasible(200, 300, 7),
+                opts(1e-9, iteration_limit=700, termination_check_frequency=50, polishing_iterations=400)))
+    return out
+
+
+def main() -> None:
+    if not ol.have_ref():
+        raise SystemExit("oracle/_ref/libref_pdlp.so is not built (make -C oracle ref)")
+    lp = box_lp(2000, 5000, 5)
+    r = ol.solve(lp, opts(1e-4), "ref")
+    np.savez_compressed(OUT / "c0_ref.npz", x=r.x, y=r.y, sha256=np.array(lp.sha256()),
+                        summary=np.array(json.dumps({k: r.d[k] for k in KEYS})))
+    table = {}
+    for name, plp, o in cases():
+        s = ol.solve(plp, o, "ref")
+        table[name] = {k: s.d[k] for k in KEYS}
+        table[name]["x"] = s.x.tolist() if plp.n <= 16 else None
+    (OUT / "test_problems_ref.json").write_text(json.dumps(table, indent=1, sort_keys=True))
+    hashes = {"c0_box_2000x5000": lp.sha256(),
+              "zipf_3000x7000_s5": zipf_lp(3000, 7000, seed=5).sha256(),
+              "zipf_20000x50000_s11": zipf_lp(20000, 50000, seed=11).sha256()}
+    (OUT / "instance_hashes.json").write_text(json.dumps(hashes, indent=1, sort_keys=True))
+    print("wrote", sorted(p.name for p in OUT.iterdir()))
+
+
+if __name__ == "__main__":
+    main()

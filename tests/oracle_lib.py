@@ -1,0 +1,147 @@
+"""Bindings of the CPU checker for the tests: oracle/liboracle.so (serial
+restatement of the reference PDLP) and, when built, oracle/_ref/libref_pdlp.so
+(the reference compiled from its own sources behind a C shim).  Test
+infrastructure only."""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from optimisation_solver_amd import _native
+from optimisation_solver_amd.lp import Lp
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_SO = ROOT / "oracle" / "_ref" / "libref_pdlp.so"
+TRACE_WIDTH = 8
+
+vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+P = C.POINTER(_native.Problem)
+O = C.POINTER(_native.Options)
+S = C.POINTER(_native.Summary)
+
+
+def _bind(lib, prefix: str):
+    getattr(lib, f"{prefix}_matrix_from_triplets").argtypes = [i32, i32, i64, vp, vp, vp]
+    getattr(lib, f"{prefix}_matrix_from_triplets").restype = vp
+    getattr(lib, f"{prefix}_matrix_nnz").argtypes = [vp]
+    getattr(lib, f"{prefix}_matrix_nnz").restype = i64
+    getattr(lib, f"{prefix}_matrix_export").argtypes = [vp] * 7
+    getattr(lib, f"{prefix}_matrix_export").restype = None
+    getattr(lib, f"{prefix}_matrix_free").argtypes = [vp]
+    getattr(lib, f"{prefix}_matrix_free").restype = None
+    getattr(lib, f"{prefix}_spmv").argtypes = [P, i32, vp, vp]
+    getattr(lib, f"{prefix}_scaling").argtypes = [P, i32, i32, vp, vp, vp, vp]
+    getattr(lib, f"{prefix}_evaluate").argtypes = [P, O, vp, vp, vp]
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        lib = C.CDLL(str(ORACLE_SO))
+        _bind(lib, "oracle")
+        lib.oracle_solve.argtypes = [P, O, vp, vp, S, vp, vp, vp]
+        lib.oracle_solve_traced.argtypes = [P, O, vp, vp, S, vp, vp, vp, vp, i32, C.POINTER(i32)]
+        lib.oracle_step_judge.argtypes = [O, f64, f64, f64, f64, f64, f64, i64, C.POINTER(f64)]
+        lib.oracle_reweight.argtypes = [O, f64, f64, f64]
+        lib.oracle_reweight.restype = f64
+        lib.oracle_restart_choose.argtypes = [O, f64, f64, f64, f64, i64, i64, C.POINTER(f64), C.POINTER(f64)]
+        _oracle = lib
+    return _oracle
+
+
+def have_ref() -> bool:
+    return REF_SO.exists()
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = C.CDLL(str(REF_SO))
+        _bind(lib, "ref")
+        lib.ref_solve.argtypes = [P, O, S, vp, vp, vp]
+        _ref = lib
+    return _ref
+
+
+class Solved:
+    def __init__(self, summary: _native.Summary, x, y, ray, trace=None):
+        self.s = summary
+        self.d = summary.as_dict()
+        self.x, self.y, self.ray, self.trace = x, y, ray, trace
+
+    def __getattr__(self, k):
+        return self.d[k]
+
+
+def solve(lp: Lp, options, which: str = "oracle", warm_x=None, warm_y=None, trace_cap: int = 0) -> Solved:
+    prob = lp.problem()
+    s = _native.Summary()
+    x, y = np.zeros(lp.n), np.zeros(lp.m)
+    ray = np.zeros(max(lp.n, lp.m, 1))
+    if which == "ref":
+        assert warm_x is None and warm_y is None
+        ref().ref_solve(C.byref(prob), C.byref(options), C.byref(s), x.ctypes.data, y.ctypes.data, ray.ctypes.data)
+        return Solved(s, x, y, ray)
+    wx = None if warm_x is None else np.ascontiguousarray(warm_x, dtype=float)
+    wy = None if warm_y is None else np.ascontiguousarray(warm_y, dtype=float)
+    tr = np.zeros((max(trace_cap, 1), TRACE_WIDTH))
+    tl = i32(0)
+    oracle().oracle_solve_traced(C.byref(prob), C.byref(options), None if wx is None else wx.ctypes.data,
+                                 None if wy is None else wy.ctypes.data, C.byref(s), x.ctypes.data,
+                                 y.ctypes.data, ray.ctypes.data, tr.ctypes.data, trace_cap, C.byref(tl))
+    return Solved(s, x, y, ray, tr[: min(tl.value, trace_cap)] if trace_cap else None)
+
+
+def spmv(lp: Lp, v, transpose: bool = False, which: str = "oracle") -> np.ndarray:
+    lib = oracle() if which == "oracle" else ref()
+    fn = getattr(lib, f"{which}_spmv")
+    prob = lp.problem()
+    v = np.ascontiguousarray(v, dtype=float)
+    out = np.zeros(lp.n if transpose else lp.m)
+    fn(C.byref(prob), int(transpose), v.ctypes.data, out.ctypes.data)
+    return out
+
+
+def scaling(lp: Lp, ruiz: int = 10, pc: bool = True, which: str = "oracle"):
+    lib = oracle() if which == "oracle" else ref()
+    fn = getattr(lib, f"{which}_scaling")
+    prob = lp.problem()
+    rs, cs, tau, sig = np.zeros(lp.m), np.zeros(lp.n), np.zeros(lp.n), np.zeros(lp.m)
+    fn(C.byref(prob), ruiz, int(pc), rs.ctypes.data, cs.ctypes.data, tau.ctypes.data, sig.ctypes.data)
+    return rs, cs, tau, sig
+
+
+def evaluate(lp: Lp, options, x, y, which: str = "oracle") -> dict:
+    lib = oracle() if which == "oracle" else ref()
+    fn = getattr(lib, f"{which}_evaluate")
+    prob = lp.problem()
+    x = np.ascontiguousarray(x, dtype=float)
+    y = np.ascontiguousarray(y, dtype=float)
+    out = np.zeros(8)
+    fn(C.byref(prob), C.byref(options), x.ctypes.data, y.ctypes.data, out.ctypes.data)
+    keys = ["primal_objective", "dual_objective", "primal_residual", "dual_residual", "relative_gap", "kkt",
+            "dual_valid", "finite"]
+    return dict(zip(keys, out.tolist()))
+
+
+def from_triplets(m, n, rows, cols, vals, which: str = "oracle"):
+    """Canonical CSR/CSC arrays through the checker's own builder."""
+    lib = oracle() if which == "oracle" else ref()
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cols = np.ascontiguousarray(cols, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=float)
+    h = getattr(lib, f"{which}_matrix_from_triplets")(m, n, rows.size, rows.ctypes.data, cols.ctypes.data,
+                                                      vals.ctypes.data)
+    nnz = getattr(lib, f"{which}_matrix_nnz")(h)
+    rp, ci, rv = np.zeros(m + 1, np.int64), np.zeros(nnz, np.int32), np.zeros(nnz)
+    cp, ri, cv = np.zeros(n + 1, np.int64), np.zeros(nnz, np.int32), np.zeros(nnz)
+    getattr(lib, f"{which}_matrix_export")(h, *(a.ctypes.data for a in (rp, ci, rv, cp, ri, cv)))
+    getattr(lib, f"{which}_matrix_free")(h)
+    return rp, ci, rv, cp, ri, cv

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP engine against the CPU oracle (a serial restatement of
+the reference PDLP, pinned bitwise to the compiled reference in
+test_oracle_pin.py) on the same seeded inputs.
+
+Tolerances: sparse products in FP64 differ from the oracle only by summation
+order, so |gpu - cpu| <= 64 eps sum_k |a_k v_k| per row; Ruiz and
+Pock-Chambolle scale vectors are exact (order-independent maxima, sums in the
+reference's order); full solves must agree to 1e-6 relative objective
+(|p_gpu - p_cpu| / (1 + |p_cpu|)) and the GPU point, re-scored by the
+oracle's KKT evaluator, must meet the requested tolerance.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import problems
+from optimisation_solver_amd import Engine, box_lp, default_options, zipf_lp
+
+pytestmark = pytest.mark.gpu
+EPS = np.finfo(float).eps
+
+
+def _bound(lp, v, transpose):
+    a = ol.spmv(lp.__class__(lp.m, lp.n, lp.rp, lp.ci, np.abs(lp.rv), lp.cp, lp.ri, np.abs(lp.cv), lp.c, lp.lo,
+                             lp.hi, lp.rlo, lp.rhi), np.abs(v), transpose)
+    return 64 * EPS * a + 1e-300
+
+
+@pytest.mark.parametrize("shape", [(25, 40, 400, 7), (3000, 2000, 36000, 11), (1, 1, 1, 1), (5, 3, 0, 2),
+                                   (4000, 300, 200000, 5)])
+def test_spmv_matches_oracle(shape):
+    m, n, nnz, seed = shape
+    lp = problems.random_matrix(m, n, nnz, seed)
+    g = np.random.Generator(np.random.PCG64(seed + 1))
+    x, y = g.uniform(-2, 2, n), g.uniform(-2, 2, m)
+    print(f"spmv case m={m} n={n} nnz={lp.nnz}", flush=True)
+    with Engine() as e:
+        e.load(lp)
+        ax, aty = e.spmv(x), e.spmv(y, transpose=True)
+    assert np.all(np.abs(ax - ol.spmv(lp, x)) <= _bound(lp, x, False))
+    assert np.all(np.abs(aty - ol.spmv(lp, y, True)) <= _bound(lp, y, True))
+    # adjoint identity <Ax, y> = <x, A^T y>
+    assert abs(ax @ y - x @ aty) <= 1e-9 * (1 + abs(ax @ y))
+
+
+def test_spmv_long_rows_and_zipf():
+    lp = zipf_lp(3000, 7000, literal=True, seed=99)   # column lengths up to 2000 > one tile
+    g = np.random.Generator(np.random.PCG64(3))
+    x, y = g.uniform(-1, 1, lp.n), g.uniform(-1, 1, lp.m)
+    with Engine() as e:
+        e.load(lp)
+        ax, aty = e.spmv(x), e.spmv(y, transpose=True)
+    assert np.all(np.abs(ax - ol.spmv(lp, x)) <= _bound(lp, x, False))
+    assert np.all(np.abs(aty - ol.spmv(lp, y, True)) <= _bound(lp, y, True))
+
+
+@pytest.mark.parametrize("maker", [lambda: box_lp(2000, 5000, 5), lambda: zipf_lp(3000, 7000, seed=5),
+                                   lambda: problems.closed_form()["ill_conditioned"][0]])
+def test_scaling_bitwise(maker):
+    lp = maker()
+    with Engine() as e:
+        e.load(lp)
+        got = e.scaling(10, True)
+    want = ol.scaling(lp, 10, True)
+    for g_, w_ in zip(got, want):
+        assert np.array_equal(g_, w_)
+
+
+def _opts(tol, **kw):
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, thread_count=1)
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _check_solution(lp, o, gpu, cpu, tol):
+    print(f"  gpu {gpu.status} its={gpu.iterations} obj={gpu.primal_objective!r}; "
+          f"cpu {cpu.status_name} its={cpu.iterations} obj={cpu.primal_objective!r}", flush=True)
+    assert gpu.status == cpu.status_name
+    rel = abs(gpu.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective))
+    assert rel <= 1e-6 if tol <= 1e-6 else rel <= 10 * tol
+    if gpu.status == "optimal":
+        m = ol.evaluate(lp, o, gpu.primal, gpu.row_dual)
+        assert m["primal_residual"] <= tol and m["dual_residual"] <= tol and m["relative_gap"] <= tol
+
+
+@pytest.mark.parametrize("name", list(problems.closed_form()))
+def test_closed_form_solves(name):
+    lp, obj, x, _ = problems.closed_form()[name] + (None,)
+    tol = 1e-8 if name == "ill_conditioned" else 2e-6
+    o = _opts(tol, iteration_limit=100000, termination_check_frequency=50, polishing_iterations=2000)
+    print(f"closed form {name}", flush=True)
+    with Engine() as e:
+        e.load(lp)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, o)
+    _check_solution(lp, o, gpu, cpu, tol)
+    assert abs(gpu.primal_objective - obj) <= (1e-5 if name == "ill_conditioned" else 1e-3)
+    if x is not None:
+        assert np.allclose(gpu.primal, x, atol=2e-4)
+    if name == "inactive_row":
+        assert gpu.row_dual[1] == 0.0
+
+
+def test_c0_matches_oracle():
+    lp = box_lp(2000, 5000, 5)
+    o = _opts(1e-4)
+    with Engine() as e:
+        e.load(lp)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, o)
+    _check_solution(lp, o, gpu, cpu, 1e-4)
+    assert abs(gpu.iterations - cpu.iterations) <= max(200, 0.25 * cpu.iterations)
+
+
+@pytest.mark.parametrize("seed", [21, 33])
+def test_random_feasible_tight(seed):
+    lp = problems.random_feasible(300, 420, seed)
+    o = _opts(1e-8, iteration_limit=60000, termination_check_frequency=40, use_feasibility_polishing=0)
+    with Engine() as e:
+        e.load(lp)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, o)
+    _check_solution(lp, o, gpu, cpu, 1e-8)
+    assert gpu.final_step_size > gpu.static_step_bound
+    assert gpu.step_trials >= gpu.iterations
+
+
+def test_infeasible_and_unbounded_statuses():
+    o = _opts(2e-6, iteration_limit=20000, termination_check_frequency=50, polishing_iterations=2000)
+    with Engine() as e:
+        e.load(problems.infeasible_rows())
+        r = e.solve(o)
+    assert r.status == "infeasible" and r.dual_ray is not None
+    y = r.dual_ray
+    red = y[0] + y[1]
+    box = (0.0 if red > 0 else red * 10.0) * 2
+    support = (0.0 if y[0] > 0 else y[0] * 2.0) + (y[1] * 1.0 if y[1] > 0 else 0.0)
+    assert box - support > 1e-9
+    with Engine() as e:
+        e.load(problems.infeasible_equalities())
+        assert e.solve(o).status == "infeasible"
+    with Engine() as e:
+        e.load(problems.unbounded())
+        r = e.solve(o)
+    assert r.status == "unbounded" and r.primal_ray[0] > 0
+    o2 = _opts(2e-6, iteration_limit=2000, termination_check_frequency=50, polishing_iterations=2000)
+    with Engine() as e:
+        e.load(problems.unbounded_no_ray_rows())
+        assert e.solve(o2).status != "optimal"
+
+
+def test_invalid_problem_and_options():
+    lp = problems.lp(1, 1, [(0, 0, 1.0)], [1], [0], [1], [5], [1])   # crossed rows
+    with Engine() as e:
+        e.load(lp)
+        r = e.solve(_opts(1e-6))
+    assert r.status == "invalid_problem" and r.status_message == "Invalid row bounds"
+    lp2 = problems.closed_form()["equality_row"][0]
+    with Engine() as e:
+        e.load(lp2)
+        r = e.solve(_opts(1e-6, termination_check_frequency=0))
+    assert r.status == "invalid_problem" and r.status_message == "Invalid PDLP options"
