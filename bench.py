@@ -1,0 +1,177 @@
+#!/usr/bin/env python3
+"""Benchmark: time to 1e-6 relative KKT of the MI355X PDLP engine on one GPU.
+
+A step is one complete solve (set-up, PDHG iterations, checks) of the seeded
+LP that is already resident in HBM (pdlpx_load runs before the timed region).
+Default workload: the power-law LP (m=200,000, n=500,000, ~5.3M nnz, FP64,
+1e-6); --config 2 selects the 1M x 2M FP32 mixed-precision LP, --config 0 the
+CPU-runnable 2,000 x 5,000 LP.
+
+Printed (stdout, one JSON line): the contract fields plus
+  roofline     the dominant trial kernel's algorithmic bytes per launch over
+               its average launch time (hipEvents on the engine's stream),
+               against the 8 TB/s HBM peak;
+  cpu_baseline the reference PDLP compiled from its own sources (oracle/_ref)
+               timed on this host's cores on a bounded sample: set-up plus N
+               iterations, scaled to the GPU solve's iteration count;
+  parity       the GPU point re-scored by the CPU oracle's KKT evaluator.
+Progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "time to 1e-6 rel. KKT & SpMV HBM GB/s; FP32+warm-start vs plain FP64 PDLP"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def trial_bytes(lp, s_v: int, s_x: int, s_d: int, s_i: int = 4, s_p: int = 4) -> tuple[float, float]:
+    """Algorithmic bytes of one launch of the primal (A^T y + prox, CSC
+    orientation) and dual (A x' + prox, CSR orientation) trial kernels.  Each
+    compulsory byte once; gathered vectors once.
+      primal: Z(s_v+s_i) + (n+1)s_p + m s_x (y) + 4n s_d (c,l,u,T) + 2n s_x (x, x')
+      dual:   Z(s_v+s_i) + (m+1)s_p + n s_x (x') + 3m s_d (l,u,Sigma)
+              + 4m s_x (y, Ax, y', Ax')"""
+    z, m, n = lp.nnz, lp.m, lp.n
+    primal = z * (s_v + s_i) + (n + 1) * s_p + m * s_x + 4 * n * s_d + 2 * n * s_x
+    dual = z * (s_v + s_i) + (m + 1) * s_p + n * s_x + 3 * m * s_d + 4 * m * s_x
+    return float(primal), float(dual)
+
+
+def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads: int) -> dict | None:
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+    from optimisation_solver_amd import default_options
+
+    kind = "reference" if ol.have_ref() else "port"
+    which = "ref" if kind == "reference" else "oracle"
+    used = threads if kind == "reference" else 1
+
+    def run(limit):
+        o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, iteration_limit=limit,
+                            thread_count=used, use_feasibility_polishing=0)
+        t = time.perf_counter()
+        r = ol.solve(lp, o, which)
+        return time.perf_counter() - t, r
+
+    t0, _ = run(0)
+    t1, r1 = run(sample_iters)
+    per_iter = max(t1 - t0, 1e-9) / max(r1.iterations, 1)
+    est = t0 + per_iter * gpu_iterations
+    return {"value": round(est, 4), "unit": "s", "cores": used, "kind": kind,
+            "sample": (f"{'reference PDLP (oracle/_ref, compiled from the reference sources)' if kind == 'reference' else 'serial restatement (oracle/)'}"
+                       f" on the same instance, {used} thread(s): set-up {t0:.3f}s (Ruiz+preconditioner+first KKT)"
+                       f" + {r1.iterations} PDHG iterations in {t1 - t0:.3f}s = {per_iter * 1e3:.3f} ms/iteration,"
+                       f" scaled to the {gpu_iterations} iterations the GPU solve needed"),
+            "ms_per_iteration": round(per_iter * 1e3, 4), "setup_s": round(t0, 4)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--cpu-sample-iters", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
+        return 2
+
+    from optimisation_solver_amd import CONFIGS, Engine, default_options
+    from optimisation_solver_amd import _native
+
+    cfg = CONFIGS[args.config]
+    t = time.perf_counter()
+    lp = cfg["make"]()
+    log(f"[bench] generated {cfg['name']}: m={lp.m} n={lp.n} nnz={lp.nnz} in {time.perf_counter() - t:.1f}s")
+    tol = cfg["tol"]
+    fp32 = args.config == 2
+    prec = _native.FP32 if fp32 else _native.FP64
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=prec,
+                        iteration_limit=10_000_000)
+
+    eng = Engine(0)
+    eng.load(lp, prec)
+    for w in range(args.warmup):
+        r = eng.solve(o)
+        log(f"[bench] warmup {w}: {r.status} its={r.iterations} t={r.solve_time_seconds:.3f}s")
+    times, res = [], None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        res = eng.solve(o)
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {k}: {res.status} its={res.iterations} trials={res.step_trials} "
+            f"restarts={res.restart_count} t={times[-1]:.3f}s kkt=({res.primal_residual:.2e},"
+            f"{res.dual_residual:.2e},{res.relative_gap:.2e}) fp32_its={res.diagnostics['fp32_iterations']}")
+    sec = sum(times) / len(times)
+
+    # dominant kernel: per-launch time with hipEvents on the engine's stream
+    phase_prec = _native.FP64 if (not fp32 or res.diagnostics["fp32_iterations"] < res.iterations) else _native.FP32
+    ms_p, ms_d = eng.time_trial(_native.FP32 if fp32 else _native.FP64, args.kernel_reps)
+    sv = 4 if fp32 else 8
+    bp, bd = trial_bytes(lp, sv, sv, sv)
+    gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
+    dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{cfg['name']}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get(dom[0])
+    roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
+            "bytes_per_launch": int(dom[2]), "ms_per_launch": round(dom[1], 5),
+            "other_kernel": {"name": "pdhg_dual_kernel" if dom[0] == "pdhg_primal_kernel" else "pdhg_primal_kernel",
+                             "ms_per_launch": round(ms_d if dom[0] == "pdhg_primal_kernel" else ms_p, 5),
+                             "achieved": round(gbs_d if dom[0] == "pdhg_primal_kernel" else gbs_p, 1)}}
+    del phase_prec
+
+    # parity: re-score the GPU point with the oracle's KKT evaluator
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+    ev = ol.evaluate(lp, o, res.primal, res.row_dual)
+    parity = {"oracle_kkt": ev["kkt"], "oracle_primal_residual": ev["primal_residual"],
+              "oracle_dual_residual": ev["dual_residual"], "oracle_gap": ev["relative_gap"],
+              "objective": res.primal_objective}
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        log(f"[bench] cpu baseline sample ({args.cpu_sample_iters} iterations, {threads} threads)")
+        cpu = cpu_baseline(lp, tol, res.iterations, args.cpu_sample_iters, threads)
+        log(f"[bench] cpu: {cpu}")
+    eng.close()
+
+    line = {
+        "metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if fp32 else "f64", "data": "synthetic",
+        "config": {"workload": cfg["name"], "m": lp.m, "n": lp.n, "nnz": lp.nnz, "tolerance": tol,
+                   "status": res.status, "iterations": res.iterations, "step_trials": res.step_trials,
+                   "restarts": res.restart_count, "fp32_iterations": res.diagnostics["fp32_iterations"],
+                   "device_loop_s": round(res.diagnostics["device_loop_seconds"], 4),
+                   "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name},
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = round(cpu["value"] / sec, 2)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

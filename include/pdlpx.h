@@ -197,6 +197,16 @@ int pdlpx_scaling(pdlpx_ctx* ctx, int32_t ruiz_iterations, int32_t use_precondit
                   double* primal_scale, double* dual_scale);
 
 /*
+ * Times the two PDHG trial kernels on the state the last pdlpx_solve left on
+ * the device: `reps` back-to-back launches of the primal half (A^T y + prox)
+ * and of the dual half (A x' + prox), each bracketed by hipEvents on the
+ * context's stream.  Writes the average milliseconds per launch.  A profiling
+ * aid with no reference counterpart; the iterate is not modified.
+ */
+int pdlpx_time_trial(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
+                     double* ms_primal, double* ms_dual);
+
+/*
  * Solves `count` independent LPs (problems[i]) in one batched device run.
  * Per-instance outputs: summaries[i]; x_out[i] (cols_i) and y_out[i] (rows_i)
  * may be NULL.  Options apply to every instance.

@@ -1279,6 +1279,59 @@ int pdlpx_scaling(pdlpx_ctx* X, int32_t ruiz_iterations, int32_t use_pc, double*
     });
 }
 
+int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_primal, double* ms_dual) {
+    if (!ms_primal || !ms_dual || reps <= 0) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded || !X->valid) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        if (precision == PDLPX_FP32 && !X->w32.vr.p) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
+        hipStream_t s = X->stream;
+        DevState saved{}, probe{};
+        HIPCHK(hipMemcpyAsync(&saved, X->st.p, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        probe = saved;
+        probe.done = 0;
+        HIPCHK(hipMemcpyAsync(X->st.p, &probe, sizeof(DevState), hipMemcpyHostToDevice, s));
+        auto timeIt = [&](auto&& launch) {
+            HIPCHK(hipEventRecord(X->ev0, s));
+            for (int r = 0; r < reps; ++r) launch();
+            HIPCHK(hipEventRecord(X->ev1, s));
+            HIPCHK(hipEventSynchronize(X->ev1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, X->ev0, X->ev1));
+            return static_cast<double>(ms) / reps;
+        };
+        if (precision == PDLPX_FP32) {
+            Iterates<float> it = X->it32.view();
+            const Working<float>& w = X->w32;
+            ColData<float> cd{w.c.p, w.lo.p, w.hi.p, w.tau.p, w.tau_inv.p};
+            RowData<float> rd{w.rlo.p, w.rhi.p, w.sig.p, w.sig_inv.p};
+            *ms_primal = timeIt([&] {
+                k_primal<float><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<float>(w.vc.p), cd, it, X->st.p, X->part_p.p);
+            });
+            *ms_dual = timeIt([&] {
+                k_dual<float><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<float>(w.vr.p), rd, it, X->st.p, X->part_d.p,
+                                                             X->part_i.p);
+            });
+        } else {
+            Iterates<double> it = X->it64.view();
+            const Working<double>& w = X->w64;
+            ColData<double> cd{w.c.p, w.lo.p, w.hi.p, w.tau.p, w.tau_inv.p};
+            RowData<double> rd{w.rlo.p, w.rhi.p, w.sig.p, w.sig_inv.p};
+            *ms_primal = timeIt([&] {
+                k_primal<double><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<double>(w.vc.p), cd, it, X->st.p,
+                                                                 X->part_p.p);
+            });
+            *ms_dual = timeIt([&] {
+                k_dual<double><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<double>(w.vr.p), rd, it, X->st.p,
+                                                              X->part_d.p, X->part_i.p);
+            });
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(X->st.p, &saved, sizeof(DevState), hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    });
+}
+
 int pdlpx_solve_batch(pdlpx_ctx* X, int32_t count, const pdlpx_problem* problems, const pdlpx_options* options,
                       pdlpx_summary* summaries, double* const* x_out, double* const* y_out) {
     if (count < 0 || (count > 0 && (!problems || !summaries || !options))) return PDLPX_E_INVALID;

@@ -1,0 +1,73 @@
+"""CPU checks of the C ABI library: it loads without a GPU, exports every
+function include/pdlpx.h declares, and the ctypes struct layouts match the
+header's field lists."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+from optimisation_solver_amd import _native
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = (ROOT / "include" / "pdlpx.h").read_text()
+
+
+def declared_functions():
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(pdlpx_\w+)\s*\(", HEADER, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_functions()
+    assert "pdlpx_solve" in names and "pdlpx_load" in names and len(names) >= 10
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_native.ENGINE_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (\w+)", out))
+    missing = [n for n in names if n not in exported]
+    assert not missing, missing
+    lib = _native.engine()   # loads without a GPU
+    for n in names:
+        assert hasattr(lib, n)
+
+
+def test_cpp_api_symbols_exported():
+    out = subprocess.run(["nm", "-DC", "--defined-only", str(_native.ENGINE_PATH)], capture_output=True,
+                         text=True, check=True).stdout
+    for sym in ["pdlp::PdlpSolver::solve(pdlp::CompiledLp const&, pdlp::PdlpOptions const&) const",
+                "pdlp::SparseMatrix::fromTriplets", "pdlp::RuizScaler::equilibrate", "pdlp::toString"]:
+        assert sym in out, sym
+
+
+def _struct_fields(name):
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), HEADER, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return re.findall(r"(\w+)(?:\[\d+\])?;", body)
+
+
+def test_struct_layouts_match_header():
+    for cname, py in [("pdlpx_problem", _native.Problem), ("pdlpx_options", _native.Options),
+                      ("pdlpx_summary", _native.Summary)]:
+        assert _struct_fields(cname) == [f[0] for f in py._fields_], cname
+
+
+def test_default_options_are_the_reference_defaults():
+    o = _native.Options()
+    _native.engine().pdlpx_default_options(C.byref(o))
+    # pdlp_engine/include/pdlp/pdlp_options.h:7-75
+    assert (o.iteration_limit, o.termination_check_frequency, o.ruiz_iterations, o.maximum_step_trials) == \
+        (200000, 100, 10, 60)
+    assert (o.primal_tolerance, o.dual_tolerance, o.gap_tolerance) == (1e-6, 1e-6, 1e-6)
+    assert (o.minimum_restart_iterations, o.maximum_restart_iterations, o.polishing_iterations) == (64, 100000, 5000)
+    assert (o.restart_sufficient_decay, o.restart_necessary_decay, o.restart_artificial_fraction) == (0.2, 0.8, 0.36)
+    assert (o.initial_step_safety, o.minimum_step_scale, o.infeasibility_tolerance) == (0.9, 1e-6, 1e-8)
+    assert all(getattr(o, f) == 1 for f in ["use_ruiz_scaling", "detect_infeasibility",
+                                             "use_diagonal_preconditioning", "use_adaptive_linesearch",
+                                             "use_averaging", "use_restarts", "use_feasibility_polishing"])
+
+
+def test_create_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        return
+    ctx = C.c_void_p()
+    rc = _native.engine().pdlpx_create(C.byref(ctx), 0)
+    assert rc != 0 and not ctx.value
