@@ -87,6 +87,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--iteration-limit", type=int, default=10_000_000,
+                    help="cap on PDHG iterations (profiling runs; a capped run is not a time-to-tolerance)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
@@ -104,7 +106,7 @@ def main() -> int:
     fp32 = args.config == 2
     prec = _native.FP32 if fp32 else _native.FP64
     o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=prec,
-                        iteration_limit=10_000_000)
+                        iteration_limit=args.iteration_limit)
 
     eng = Engine(0)
     eng.load(lp, prec)
@@ -166,6 +168,7 @@ def main() -> int:
                    "device_loop_s": round(res.diagnostics["device_loop_seconds"], 4),
                    "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
+        "capped": args.iteration_limit < 10_000_000,
     }
     if cpu:
         line["speedup_vs_cpu"] = round(cpu["value"] / sec, 2)

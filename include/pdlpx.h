@@ -166,6 +166,25 @@ const char* pdlpx_last_error(const pdlpx_ctx* ctx);
 int pdlpx_load(pdlpx_ctx* ctx, const pdlpx_problem* problem, int32_t precision);
 
 /*
+ * Loads `count` independent LPs as one device batch (back to back in HBM).
+ * Invalid instances are kept aside and reported as INVALID_PROBLEM by the
+ * solve; the others are solved together.  No reference counterpart: the
+ * batched form of pdlpx_load.
+ */
+int pdlpx_load_batch(pdlpx_ctx* ctx, int32_t count, const pdlpx_problem* problems, int32_t precision);
+
+/*
+ * Solves every loaded instance in one batched device run: each trial slot of
+ * kernels covers the whole batch, each instance keeps its own step size,
+ * restarts and termination.  Per-instance arrays indexed like the load:
+ * warm_x / warm_y (NULL, or entries that may be NULL), summaries[count],
+ * x_out / y_out (NULL, or entries that may be NULL).
+ */
+int pdlpx_solve_loaded(pdlpx_ctx* ctx, const pdlpx_options* options,
+                       const double* const* warm_x, const double* const* warm_y,
+                       pdlpx_summary* summaries, double* const* x_out, double* const* y_out);
+
+/*
  * Solves the loaded LP.  warm_x (cols) / warm_y (rows) are optional host start
  * points in the caller's coordinates (NULL = the reference's cold start).
  * x_out (cols) and y_out (rows) receive the returned candidate in the
@@ -207,7 +226,7 @@ int pdlpx_time_trial(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
                      double* ms_primal, double* ms_dual);
 
 /*
- * Solves `count` independent LPs (problems[i]) in one batched device run.
+ * pdlpx_load_batch followed by pdlpx_solve_loaded without warm starts.
  * Per-instance outputs: summaries[i]; x_out[i] (cols_i) and y_out[i] (rows_i)
  * may be NULL.  Options apply to every instance.
  */

@@ -100,6 +100,8 @@ def engine() -> C.CDLL:
         lib.pdlpx_spmv.argtypes = [vp, i32, vp, vp]
         lib.pdlpx_spmv_device.argtypes = [vp, i32, vp, vp]
         lib.pdlpx_scaling.argtypes = [vp, i32, i32, vp, vp, vp, vp]
+        lib.pdlpx_load_batch.argtypes = [vp, i32, C.POINTER(Problem), i32]
+        lib.pdlpx_solve_loaded.argtypes = [vp, C.POINTER(Options), vp, vp, C.POINTER(Summary), vp, vp]
         lib.pdlpx_time_trial.argtypes = [vp, i32, i32, pf64, pf64]
         lib.pdlpx_solve_batch.argtypes = [vp, i32, C.POINTER(Problem), C.POINTER(Options),
                                           C.POINTER(Summary), vp, vp]

@@ -26,7 +26,7 @@ ARCH = os.environ.get("PDLP_OFFLOAD_ARCH", "gfx950")
 # -ffp-contract=off keeps the FP64 epilogues in the reference's rounding;
 # packed FP32 instructions are disabled on purpose for this toolchain.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{ROOT / 'include'}",
     "-Wall", "-Wextra", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

@@ -26,10 +26,11 @@ constexpr int kReduceThreads = 1024;   // single-workgroup final reductions
 // Matrix orientation as the kernels see it: rows of this orientation.
 template <typename T>
 struct Orient {
-    const int32_t* ptr;   // nrows + 1
-    const int32_t* idx;   // nnz
-    const T* val;         // nnz
-    const int32_t* blk;   // nblk + 1 row-block starts
+    const int32_t* ptr;     // nrows + 1
+    const int32_t* idx;     // nnz
+    const T* val;           // nnz
+    const int32_t* blk;     // nblk + 1 row-block starts
+    const int32_t* binst;   // nblk: batch instance owning each block
     int32_t nblk;
     int32_t nrows;
 };

@@ -1,19 +1,21 @@
 // MI355X PDLP engine: the C ABI of include/pdlpx.h over hand-written gfx950
 // kernels (kernels.hpp).
 //
-// One solve keeps the LP, the scaled working copy and every iterate resident
-// in HBM.  The PDHG iteration runs as "trial slots" of four kernels --
-// primal half, dual half, linesearch decision, averaging blend -- whose
-// accept/reject and buffer swap are decided on the device (DevState), so a
-// batch of up to one termination interval of iterations runs with no host
-// round trip; the batch is captured once into a hipGraph and replayed.  The
-// host only wakes at termination checks (every terminationCheckFrequency
-// iterations, pdlp_solver.cpp:350-476) to read a few reduced scalars, decide
-// restarts and best candidates, and launch the next batch.
+// A context holds a batch of B LPs (B = 1 for pdlpx_solve) back to back in
+// HBM: the original matrix in CSR and CSC, the scaled working copy, and every
+// iterate.  The PDHG iteration runs as "trial slots" of four kernels --
+// primal half, dual half, linesearch decision, averaging blend -- each covering
+// the whole batch.  Accept/reject and the buffer swap are decided on the
+// device per instance (DevState), so up to one termination interval of
+// iterations runs with no host round trip; the slot sequence is captured once
+// into a hipGraph and replayed.  The host wakes at termination checks
+// (every terminationCheckFrequency iterations, pdlp_solver.cpp:350-476) to read
+// a few reduced scalars per instance, decide restarts and best candidates, and
+// launch the next interval.
 
 #include "kernels.hpp"
 
-#include "../../include/pdlpx.h"
+#include "pdlpx.h"
 
 #include <hip/hip_runtime.h>
 
@@ -33,7 +35,6 @@ using namespace pdlpx_dev;
 namespace {
 
 constexpr double kInf = std::numeric_limits<double>::infinity();
-constexpr int kStreamBlocks = 512;   // grid of the streaming reduction kernels
 
 struct Failure {
     int code;
@@ -62,10 +63,9 @@ struct DBuf {
         n = 0;
     }
     void alloc(size_t count) {
-        if (count == n && p) return;
+        if (count <= n && p) return;
         release();
-        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-        HIPCHK(hipMalloc(&p, bytes));
+        HIPCHK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
         n = count;
     }
     void upload(const T* src, size_t count, hipStream_t s) {
@@ -77,42 +77,77 @@ struct DBuf {
 int gridFor(int64_t n, int threads = kThreads) {
     return static_cast<int>(std::max<int64_t>(1, (n + threads - 1) / threads));
 }
-int streamGrid(int64_t n) { return std::min(kStreamBlocks, gridFor(n)); }
 
 // Row blocks of one orientation: at most kTileNnz nonzeros and kTileRows rows
-// each; a longer row gets a block of its own.
-std::vector<int32_t> partitionRows(const std::vector<int32_t>& ptr, int nrows) {
-    std::vector<int32_t> blk;
-    blk.push_back(0);
-    int r = 0;
-    while (r < nrows) {
-        const int start = r;
-        const int32_t base = ptr[r];
-        if (ptr[r + 1] - base > kTileNnz) {
-            blk.push_back(++r);
-            continue;
+// each, never straddling two instances; a longer row gets a block of its own.
+// ibeg[k] = first block of instance k.
+void partitionRows(const std::vector<int32_t>& ptr, const std::vector<int32_t>& instStart,
+                   std::vector<int32_t>& blk, std::vector<int32_t>& binst, std::vector<int32_t>& ibeg) {
+    const int B = static_cast<int>(instStart.size()) - 1;
+    blk.assign(1, 0);
+    binst.clear();
+    ibeg.assign(1, 0);
+    for (int k = 0; k < B; ++k) {
+        int r = instStart[k];
+        const int end = instStart[k + 1];
+        int made = 0;
+        while (r < end) {
+            const int start = r;
+            const int32_t base = ptr[r];
+            if (ptr[r + 1] - base > kTileNnz) {
+                ++r;
+            } else {
+                while (r < end && r - start < kTileRows && ptr[r + 1] - base <= kTileNnz) ++r;
+            }
+            blk.push_back(r);
+            binst.push_back(k);
+            ++made;
         }
-        while (r < nrows && r - start < kTileRows && ptr[r + 1] - base <= kTileNnz) ++r;
-        blk.push_back(r);
+        if (made == 0) {   // an instance without rows here keeps one empty block
+            blk.push_back(end);
+            binst.push_back(k);
+        }
+        ibeg.push_back(static_cast<int32_t>(binst.size()));
     }
-    if (blk.size() == 1) blk.push_back(0);   // at least one (empty) block
-    return blk;
+}
+
+// Streaming segments of at most kSegLen elements inside each instance range;
+// ibeg[k] = first segment of instance k (counted from the first segment added).
+void makeSegments(const std::vector<int32_t>& instStart, std::vector<Seg>& segs, std::vector<int32_t>& ibeg) {
+    const int B = static_cast<int>(instStart.size()) - 1;
+    const int32_t first = static_cast<int32_t>(segs.size());
+    ibeg.assign(1, 0);
+    for (int k = 0; k < B; ++k) {
+        int a = instStart[k];
+        const int e = instStart[k + 1];
+        int made = 0;
+        while (a < e) {
+            const int b = std::min(e, a + kSegLen);
+            segs.push_back(Seg{k, a, b, 0});
+            a = b;
+            ++made;
+        }
+        if (!made) segs.push_back(Seg{k, e, e, 0});
+        ibeg.push_back(static_cast<int32_t>(segs.size()) - first);
+    }
 }
 
 struct OrientBuf {
-    DBuf<int32_t> ptr, idx, blk;
+    DBuf<int32_t> ptr, idx, blk, binst, ibeg;
     DBuf<double> val;   // original values
     int32_t nblk = 0, nrows = 0;
     template <typename T>
     Orient<T> view(const T* values) const {
-        return Orient<T>{ptr.p, idx.p, values, blk.p, nblk, nrows};
+        return Orient<T>{ptr.p, idx.p, values, blk.p, binst.p, nblk, nrows};
     }
 };
 
 template <typename T>
 struct Working {   // scaled problem data in precision T
     DBuf<T> vr, vc;   // CSR / CSC values
-    DBuf<T> c, lo, hi, tau, tau_inv, rlo, rhi, sig, sig_inv;
+    DBuf<T> c, lo, hi, tau, rlo, rhi, sig;
+    ColData<T> cols() const { return ColData<T>{c.p, lo.p, hi.p, tau.p}; }
+    RowData<T> rows() const { return RowData<T>{rlo.p, rhi.p, sig.p}; }
 };
 
 template <typename T>
@@ -129,8 +164,8 @@ struct IterBuf {
         anx.alloc(n);
         any.alloc(m);
     }
-    Iterates<T> view() {
-        return Iterates<T>{{x[0].p, x[1].p}, {y[0].p, y[1].p}, {ax[0].p, ax[1].p}, avx.p, avy.p};
+    Iterates<T> view() const {
+        return Iterates<T>{{x[0].p, x[1].p}, {y[0].p, y[1].p}, {ax[0].p, ax[1].p}, avx.p, avy.p, anx.p, any.p};
     }
 };
 
@@ -141,9 +176,22 @@ struct Metrics {   // termination.h:13-22
     bool dualValid = false, finite = false;
 };
 
-void setStatusText(pdlpx_summary* s, const std::string& text) {
+void setText(pdlpx_summary* s, const std::string& text) {
     std::snprintf(s->message, sizeof(s->message), "%s", text.c_str());
 }
+
+// Host-side description of one loaded instance.
+struct InstInfo {
+    bool valid = true;
+    std::string invalid;
+    int m = 0, n = 0;
+    int64_t nnz = 0;
+    double offset = 0.0;
+    double pnorm = 1.0, dnorm = 1.0;   // termination normalisers (termination.cpp:15-61)
+};
+
+// Per-instance flag arrays in ctx.flags (each B ints).
+enum Flag { F_MODE = 0, F_MODE64, F_MODE32, F_AUX64, F_AUX32, F_SEL, F_WX, F_WY, F_COUNT };
 
 }  // namespace
 
@@ -152,33 +200,37 @@ struct pdlpx_ctx {
     hipStream_t stream = nullptr;
     std::string error;
 
-    // ---- loaded problem (caller's coordinates) ----
+    // ---- loaded batch ----
     bool loaded = false;
-    bool valid = false;
-    std::string invalid;
+    std::vector<InstInfo> info;        // per caller instance
+    std::vector<int> dev;              // device instance -> caller index
+    int B = 0;                         // device instances
+    std::vector<int32_t> rowOff, colOff;
     int m = 0, n = 0;
     int64_t nnz = 0;
-    double offset = 0.0;
-    double pnorm = 1.0, dnorm = 1.0;   // termination normalisers (termination.cpp:15-61)
     OrientBuf A, AT;                   // CSR of A and CSR of A^T (= CSC of A)
     DBuf<double> c, lo, hi, rlo, rhi;
+    DBuf<Seg> segs;
+    int ncs = 0, nrs = 0;
+    DBuf<int32_t> colSegBeg, rowSegBeg, rowSegBegAll;
 
-    // ---- per-solve work (sized at load) ----
+    // ---- per-solve work ----
     DBuf<double> rs, cs, rmax, cmax;
     Working<double> w64;
     Working<float> w32;
     IterBuf<double> it64;
     IterBuf<float> it32;
-    DBuf<double> xo0, yo0, xo1, yo1, bestx, besty, rx, ry, proj, u, warmx, warmy;
-    DBuf<double> part_p, part_d, part_i, part_rows, part_cols, part_s, part_s2, part_s3;
-    DBuf<double> red;   // reduction results
+    DBuf<double> xo0, yo0, xo1, yo1, bestx, besty, pbx, pby, rx, ry, proj, u, warmx, warmy;
+    DBuf<double> part_p, part_d, part_i, part_rows, part_cols, part_seg, part_seg2, part_seg3;
+    DBuf<double> red;
+    DBuf<int32_t> flags;
     DBuf<DevState> st;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool have32 = false;
 
-    // captured trial-slot graphs, keyed on precision and slot count
     struct GraphEntry {
         hipGraphExec_t exec = nullptr;
-        int prec = -1;
+        int phases = 0;
         int slots = 0;
     };
     std::vector<GraphEntry> graphs;
@@ -196,26 +248,24 @@ namespace {
 // Validation (compiled_lp.h:32-62, sparse_matrix.cpp:203-241).
 // ---------------------------------------------------------------------------
 std::string validateProblem(const pdlpx_problem& p) {
-    if (p.rows < 0 || p.cols < 0 || p.nnz < 0) return "CompiledLp contains an invalid sparse matrix";
-    if (p.nnz > 0 && (!p.csr_col || !p.csr_val || !p.csc_row || !p.csc_val))
-        return "CompiledLp contains an invalid sparse matrix";
-    if (!p.csr_row_start || !p.csc_col_start) return "CompiledLp contains an invalid sparse matrix";
+    const std::string bad = "CompiledLp contains an invalid sparse matrix";
+    if (p.rows < 0 || p.cols < 0 || p.nnz < 0) return bad;
+    if (!p.csr_row_start || !p.csc_col_start) return bad;
+    if (p.nnz > 0 && (!p.csr_col || !p.csr_val || !p.csc_row || !p.csc_val)) return bad;
     if (p.csr_row_start[0] != 0 || p.csc_col_start[0] != 0 || p.csr_row_start[p.rows] != p.nnz ||
         p.csc_col_start[p.cols] != p.nnz)
-        return "CompiledLp contains an invalid sparse matrix";
+        return bad;
     for (int i = 0; i < p.rows; ++i)
-        if (p.csr_row_start[i + 1] < p.csr_row_start[i]) return "CompiledLp contains an invalid sparse matrix";
+        if (p.csr_row_start[i + 1] < p.csr_row_start[i]) return bad;
     for (int j = 0; j < p.cols; ++j)
-        if (p.csc_col_start[j + 1] < p.csc_col_start[j]) return "CompiledLp contains an invalid sparse matrix";
+        if (p.csc_col_start[j + 1] < p.csc_col_start[j]) return bad;
     for (int64_t k = 0; k < p.nnz; ++k) {
-        if (p.csr_col[k] < 0 || p.csr_col[k] >= p.cols || !std::isfinite(p.csr_val[k]))
-            return "CompiledLp contains an invalid sparse matrix";
-        if (p.csc_row[k] < 0 || p.csc_row[k] >= p.rows || !std::isfinite(p.csc_val[k]))
-            return "CompiledLp contains an invalid sparse matrix";
+        if (p.csr_col[k] < 0 || p.csr_col[k] >= p.cols || !std::isfinite(p.csr_val[k])) return bad;
+        if (p.csc_row[k] < 0 || p.csc_row[k] >= p.rows || !std::isfinite(p.csc_val[k])) return bad;
     }
-    if ((p.cols > 0 && (!p.objective || !p.var_lower || !p.var_upper)))
+    if (p.cols > 0 && (!p.objective || !p.var_lower || !p.var_upper))
         return "CompiledLp column vector dimensions do not match A";
-    if ((p.rows > 0 && (!p.row_lower || !p.row_upper))) return "CompiledLp row vector dimensions do not match A";
+    if (p.rows > 0 && (!p.row_lower || !p.row_upper)) return "CompiledLp row vector dimensions do not match A";
     if (!std::isfinite(p.objective_offset)) return "Objective offset must be finite";
     for (int j = 0; j < p.cols; ++j)
         if (!std::isfinite(p.objective[j]) || std::isnan(p.var_lower[j]) || std::isnan(p.var_upper[j]) ||
@@ -224,7 +274,6 @@ std::string validateProblem(const pdlpx_problem& p) {
     for (int i = 0; i < p.rows; ++i)
         if (std::isnan(p.row_lower[i]) || std::isnan(p.row_upper[i]) || p.row_lower[i] > p.row_upper[i])
             return "Invalid row bounds";
-    if (p.nnz >= (int64_t(1) << 31)) return "Matrix has too many nonzeros for 32-bit device offsets";
     return "";
 }
 
@@ -234,31 +283,16 @@ bool optionsValid(const pdlpx_options& o) {   // pdlp_solver.cpp:127-133
              o.minimum_primal_weight <= 0.0 || o.maximum_primal_weight < o.minimum_primal_weight);
 }
 
-void uploadOrient(OrientBuf& ob, int nrows, const int64_t* start, const int32_t* idx, const double* val,
-                  int64_t nnz, hipStream_t s) {
-    std::vector<int32_t> ptr(static_cast<size_t>(nrows) + 1);
-    for (int r = 0; r <= nrows; ++r) ptr[r] = static_cast<int32_t>(start[r]);
-    const std::vector<int32_t> blk = partitionRows(ptr, nrows);
-    ob.nrows = nrows;
-    ob.nblk = static_cast<int32_t>(blk.size()) - 1;
-    ob.ptr.upload(ptr.data(), ptr.size(), s);
-    ob.blk.upload(blk.data(), blk.size(), s);
-    ob.idx.upload(idx, static_cast<size_t>(nnz), s);
-    ob.val.upload(val, static_cast<size_t>(nnz), s);
-    HIPCHK(hipStreamSynchronize(s));   // host staging vectors die here
-}
-
 // ===========================================================================
-// One solve.
+// One batched solve.
 // ===========================================================================
 class Run {
 public:
-    Run(pdlpx_ctx& c, const pdlpx_options& o) : X(c), o(o), s(c.stream) {}
+    Run(pdlpx_ctx& c, const pdlpx_options& o) : X(c), o(o), s(c.stream), B(c.B) {}
 
-    void go(const double* warmX, const double* warmY, pdlpx_summary* sum, double* xOut, double* yOut,
-            double* rayOut);
+    void go(const double* const* warmX, const double* const* warmY, pdlpx_summary* const* sums,
+            double* const* xOut, double* const* yOut, double* const* rayOut);
 
-    // Runs only the set-up (scaling and step scales) and waits for it.
     void setupOnly() {
         setup();
         HIPCHK(hipStreamSynchronize(s));
@@ -268,126 +302,174 @@ private:
     pdlpx_ctx& X;
     const pdlpx_options& o;
     hipStream_t s;
+    const int B;
     using Clock = std::chrono::steady_clock;
     Clock::time_point t0 = Clock::now();
     double since() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
-    bool scaled = false;
-    int prec = PDLPX_FP64;   // precision of the current phase
-    DevState hs{};           // host mirror of the device state
-    double maxSafe = 0.99;
-    int64_t fp32Its = 0;
-    int checks = 0;
+    std::vector<DevState> hs;   // host mirror of the device states
+    std::vector<double> maxSafe;
+
+    // per-instance driver state
+    struct Inst {
+        bool mainDone = false;   // out of the main loop
+        bool finished = false;   // result final (no polishing)
+        int status = PDLPX_ITERATION_LIMIT;
+        std::string msg = "Iteration limit reached";
+        int rayKind = 0;
+        int restarts = 0;
+        bool diag = false;
+        Metrics best;
+        double baseline = kInf, prevScore = kInf;
+        int64_t sinceRestart = 0;
+        int stall = 0;
+        double lastBest = kInf;
+        int64_t fp32Its = 0;
+        int checks = 0;
+        double finishTime = 0.0;
+        bool polishing = false;
+        Metrics pBest;
+        bool pImproved = false;
+        int64_t pIts = 0, pTrials = 0, trials0 = 0;
+        int64_t mainIts = 0, mainTrials = 0;
+        double mainEta = 0, mainOmega = 1, mainSafe = 0;
+    };
+    std::vector<Inst> I;
+    std::vector<std::vector<double>> rays;   // per instance, filled when a ray is found
+    std::vector<std::vector<int32_t>> fl;    // host copies of the flag arrays
     double loopSeconds = 0.0, setupSeconds = 0.0;
 
-    // ------------------------------------------------------------------
+    int32_t* flagDev(Flag f) const { return X.flags.p + size_t(f) * B; }
+    void pushFlags(std::initializer_list<Flag> which) {
+        for (Flag f : which)
+            HIPCHK(hipMemcpyAsync(flagDev(f), fl[f].data(), sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
+    }
+    static bool anyOf(const std::vector<int32_t>& v) {
+        return std::any_of(v.begin(), v.end(), [](int32_t e) { return e != 0; });
+    }
+
     void pullState() {
-        HIPCHK(hipMemcpyAsync(&hs, X.st.p, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(hs.data(), X.st.p, sizeof(DevState) * B, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     }
-    void pushState() { HIPCHK(hipMemcpyAsync(X.st.p, &hs, sizeof(DevState), hipMemcpyHostToDevice, s)); }
+    void pushState() { HIPCHK(hipMemcpyAsync(X.st.p, hs.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s)); }
 
-    // Reduces `k` partial arrays into X.red[0..k) and returns them.
-    std::vector<double> reduce(std::initializer_list<std::pair<const double*, int>> arrays) {
-        ReduceJob job{};
-        int k = 0;
-        for (const auto& a : arrays) {
-            job.src[k] = a.first;
-            job.count[k] = a.second;
-            job.dst[k] = X.red.p + k;
-            ++k;
+    // Segmented reduction of up to 16 quantities into red[(dstOffset+q)*B + k];
+    // returns out[q][k] when toHost.
+    struct RQ {
+        const double* src;
+        const int32_t* beg;
+    };
+    std::vector<std::vector<double>> reduce(const std::vector<RQ>& qs, bool toHost = true, int dstOffset = 0) {
+        SegReduce job{};
+        const int Q = static_cast<int>(qs.size());
+        for (int q = 0; q < Q; ++q) {
+            job.src[q] = qs[q].src;
+            job.beg[q] = qs[q].beg;
+            job.dst[q] = X.red.p + size_t(dstOffset + q) * B;
         }
-        k_reduce<<<k, kReduceThreads, 0, s>>>(job);
+        k_reduce_seg<<<dim3(B, Q), kThreads, 0, s>>>(job);
         HIPCHK(hipGetLastError());
-        std::vector<double> out(static_cast<size_t>(k));
-        HIPCHK(hipMemcpyAsync(out.data(), X.red.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        std::vector<std::vector<double>> out;
+        if (!toHost) return out;
+        std::vector<double> flat(size_t(Q) * B);
+        HIPCHK(hipMemcpyAsync(flat.data(), X.red.p + size_t(dstOffset) * B, sizeof(double) * flat.size(),
+                              hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        for (int q = 0; q < Q; ++q)
+            out.emplace_back(flat.begin() + size_t(q) * B, flat.begin() + size_t(q + 1) * B);
         return out;
     }
 
     template <typename T>
-    Working<T>& work();
+    const Working<T>& work() const;
     template <typename T>
-    IterBuf<T>& iters();
+    const IterBuf<T>& iters() const;
 
-    template <typename T>
-    ColData<T> cols() {
-        Working<T>& w = work<T>();
-        return ColData<T>{w.c.p, w.lo.p, w.hi.p, w.tau.p, w.tau_inv.p};
-    }
-    template <typename T>
-    RowData<T> rows() {
-        Working<T>& w = work<T>();
-        return RowData<T>{w.rlo.p, w.rhi.p, w.sig.p, w.sig_inv.p};
-    }
+    int nsegs() const { return X.ncs + X.nrs; }
 
-    // ------------------------------------------------------------------
     void setup();
-    double powerIteration();
+    std::vector<double> powerIteration(std::vector<char>& converged);
 
+    // ---- trial slots ----
     template <typename T>
-    void launchSlot() {
-        Working<T>& w = work<T>();
-        Iterates<T> it = iters<T>().view();
-        k_primal<T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<T>(w.vc.p), cols<T>(), it, X.st.p, X.part_p.p);
-        k_dual<T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<T>(w.vr.p), rows<T>(), it, X.st.p, X.part_d.p,
-                                                 X.part_i.p);
-        k_decide<<<1, kReduceThreads, 0, s>>>(X.st.p, X.part_p.p, X.AT.nblk, X.part_d.p, X.part_i.p,
-                                               X.A.nblk);
-        k_average<T><<<streamGrid(X.n + X.m) * 4, kThreads, 0, s>>>(X.st.p, it, X.n, X.m);
+    void launchPrimal() {
+        const Working<T>& w = work<T>();
+        k_primal<T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<T>(w.vc.p), w.cols(), iters<T>().view(), X.st.p,
+                                                   X.part_p.p);
     }
-
     template <typename T>
-    hipGraphExec_t graphFor(int slots) {
-        const int key = sizeof(T) == 8 ? PDLPX_FP64 : PDLPX_FP32;
+    void launchDual() {
+        const Working<T>& w = work<T>();
+        k_dual<T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<T>(w.vr.p), w.rows(), iters<T>().view(), X.st.p,
+                                                X.part_d.p, X.part_i.p);
+    }
+    template <typename T>
+    void launchAverage() {
+        k_average<T><<<nsegs(), kThreads, 0, s>>>(X.st.p, iters<T>().view(), X.segs.p, X.ncs);
+    }
+    void launchSlot(int phases) {
+        if (phases & 1) launchPrimal<double>();
+        if (phases & 2) launchPrimal<float>();
+        if (phases & 1) launchDual<double>();
+        if (phases & 2) launchDual<float>();
+        k_decide<<<B, kThreads, 0, s>>>(X.st.p, X.AT.ibeg.p, X.A.ibeg.p, X.part_p.p, X.part_d.p, X.part_i.p);
+        if (phases & 1) launchAverage<double>();
+        if (phases & 2) launchAverage<float>();
+    }
+    hipGraphExec_t graphFor(int phases, int slots) {
         for (auto& g : X.graphs)
-            if (g.prec == key && g.slots == slots) return g.exec;
+            if (g.phases == phases && g.slots == slots) return g.exec;
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        for (int k = 0; k < slots; ++k) launchSlot<T>();
+        for (int k = 0; k < slots; ++k) launchSlot(phases);
         HIPCHK(hipStreamEndCapture(s, &graph));
         hipGraphExec_t exec = nullptr;
         const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIPCHK(e);
-        X.graphs.push_back({exec, key, slots});
+        X.graphs.push_back({exec, phases, slots});
         return exec;
     }
 
-    // Runs trials until hs.iteration reaches `target` or the device flags a
-    // failure.  The state mirror is pushed first and pulled at the end.
-    template <typename T>
-    void runTrials(int64_t target) {
-        hs.target = target;
-        hs.done = hs.iteration >= target ? 1 : 0;
+    // Runs trials until every instance with done == 0 reaches its target or
+    // fails.  Pushes the state mirror first and pulls it at the end.  Every
+    // slot either commits an iteration or counts a rejection toward the
+    // linesearch budget, so the loop ends.
+    void runTrials() {
         pushState();
-        if (hs.done) return;
+        int phases = 0;
+        int64_t left = 0;
+        for (int k = 0; k < B; ++k)
+            if (!hs[k].done) {
+                phases |= hs[k].phase ? 2 : 1;
+                left = std::max<int64_t>(left, hs[k].target - hs[k].iteration);
+            }
+        if (!phases) return;
+        const int big = o.termination_check_frequency + 4;
         while (true) {
-            const int64_t left = target - hs.iteration;
-            const int big = o.termination_check_frequency + 4;
-            int slots = left + 4 > 16 ? big : 16;
-            if (slots > big) slots = big;
-            HIPCHK(hipGraphLaunch(graphFor<T>(slots), s));
+            const int slots = static_cast<int>(std::min<int64_t>(big, left + 4 > 16 ? big : 16));
+            HIPCHK(hipGraphLaunch(graphFor(phases, slots), s));
             pullState();
-            if (hs.done) break;
+            bool any = false;
+            left = 0;
+            for (int k = 0; k < B; ++k)
+                if (!hs[k].done) {
+                    any = true;
+                    left = std::max<int64_t>(left, hs[k].target - hs[k].iteration);
+                }
+            if (!any) break;
         }
     }
 
-    template <typename T>
-    void refreshActivity() {   // A x for the current iterate (pdhg_kernel.cpp:40-42)
-        IterBuf<T>& I = iters<T>();
-        k_spmv<T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<T>(work<T>().vr.p), I.x[hs.parity].p,
-                                                 I.ax[hs.parity].p);
-        HIPCHK(hipGetLastError());
-    }
-
-    Metrics finish(double viol, double support, double rowDrift, double obj, double inf,
+    Metrics finish(int k, double viol, double support, double rowDrift, double obj, double inf,
                    double colDrift) const {   // termination.cpp:296-324
+        const InstInfo& in = X.info[X.dev[k]];
         Metrics m;
-        m.pres = std::sqrt(viol) / X.pnorm;
-        m.dres = std::sqrt(colDrift + rowDrift) / X.dnorm;
-        m.pobj = X.offset + obj;
-        m.dobj = X.offset + inf - support;
+        m.pres = std::sqrt(viol) / in.pnorm;
+        m.dres = std::sqrt(colDrift + rowDrift) / in.dnorm;
+        m.pobj = in.offset + obj;
+        m.dobj = in.offset + inf - support;
         m.finite = std::isfinite(m.pobj) && std::isfinite(m.dobj) && std::isfinite(m.pres) &&
                    std::isfinite(m.dres);
         if (!m.finite) {
@@ -405,185 +487,225 @@ private:
                m.gap <= o.gap_tolerance;
     }
 
-    // Scores the candidates already in original coordinates in xo0/yo0 and
-    // xo1/yo1 (one fused row pass and one fused column pass).
-    void scorePair(Metrics& m0, Metrics& m1) {
-        k_term_rows<<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.A.val.p), X.xo0.p, X.xo1.p, X.yo0.p,
-                                                  X.yo1.p, X.rlo.p, X.rhi.p, X.part_rows.p);
-        k_term_cols<<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.AT.val.p), X.yo0.p, X.yo1.p, X.xo0.p,
-                                                   X.xo1.p, X.c.p, X.lo.p, X.hi.p, X.part_cols.p);
+    // Scores the current (and, where fl[F_MODE] == 2, the averaged) iterate of
+    // every instance with fl[F_MODE] != 0; leaves the original-coordinate
+    // vectors in xo0/yo0 and xo1/yo1 and the phase-split modes in
+    // F_MODE64 / F_MODE32.
+    void score(std::vector<Metrics>& cur, std::vector<Metrics>& avg) {
+        std::fill(fl[F_MODE64].begin(), fl[F_MODE64].end(), 0);
+        std::fill(fl[F_MODE32].begin(), fl[F_MODE32].end(), 0);
+        for (int k = 0; k < B; ++k)
+            if (fl[F_MODE][k]) (hs[k].phase ? fl[F_MODE32] : fl[F_MODE64])[k] = fl[F_MODE][k];
+        pushFlags({F_MODE, F_MODE64, F_MODE32});
+        if (anyOf(fl[F_MODE64]))
+            k_to_original<double><<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.cs.p, X.rs.p, X.it64.view(), X.st.p,
+                                                               flagDev(F_MODE64), X.xo0.p, X.yo0.p, X.xo1.p, X.yo1.p);
+        if (anyOf(fl[F_MODE32]))
+            k_to_original<float><<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.cs.p, X.rs.p, X.it32.view(), X.st.p,
+                                                              flagDev(F_MODE32), X.xo0.p, X.yo0.p, X.xo1.p, X.yo1.p);
+        HIPCHK(hipGetLastError());
+        k_term_rows<<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.A.val.p), flagDev(F_MODE), X.xo0.p, X.xo1.p,
+                                                  X.yo0.p, X.yo1.p, X.rlo.p, X.rhi.p, X.part_rows.p);
+        k_term_cols<<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.AT.val.p), flagDev(F_MODE), X.yo0.p,
+                                                   X.yo1.p, X.xo0.p, X.xo1.p, X.c.p, X.lo.p, X.hi.p, X.part_cols.p);
         HIPCHK(hipGetLastError());
         const int nr = X.A.nblk, nc = X.AT.nblk;
-        const double* pr = X.part_rows.p;
-        const double* pc = X.part_cols.p;
-        const auto v = reduce({{pr, nr}, {pr + nr, nr}, {pr + 2 * nr, nr}, {pr + 3 * nr, nr},
-                               {pr + 4 * nr, nr}, {pr + 5 * nr, nr}, {pc, nc}, {pc + nc, nc},
-                               {pc + 2 * nc, nc}, {pc + 3 * nc, nc}, {pc + 4 * nc, nc},
-                               {pc + 5 * nc, nc}});
-        m0 = finish(v[0], v[1], v[2], v[6], v[7], v[8]);
-        m1 = finish(v[3], v[4], v[5], v[9], v[10], v[11]);
-        ++checks;
+        std::vector<RQ> q;
+        for (int i = 0; i < 6; ++i) q.push_back({X.part_rows.p + size_t(i) * nr, X.A.ibeg.p});
+        for (int i = 0; i < 6; ++i) q.push_back({X.part_cols.p + size_t(i) * nc, X.AT.ibeg.p});
+        const auto v = reduce(q);
+        cur.assign(B, Metrics{});
+        avg.assign(B, Metrics{});
+        for (int k = 0; k < B; ++k) {
+            if (!fl[F_MODE][k]) continue;
+            cur[k] = finish(k, v[0][k], v[1][k], v[2][k], v[6][k], v[7][k], v[8][k]);
+            avg[k] = finish(k, v[3][k], v[4][k], v[5][k], v[9][k], v[10][k], v[11][k]);
+            ++I[k].checks;
+        }
     }
 
-    template <typename T>
-    void toOriginal(const T* x, const T* y, double* xo, double* yo) {
-        k_to_original<T><<<streamGrid(X.n + X.m) * 4, kThreads, 0, s>>>(X.n, X.m, X.cs.p, X.rs.p, x, y, xo,
-                                                                         yo);
+    // Ray tests (infeasibility.cpp:51-192) for the instances with
+    // fl[F_MODE] != 0, on the phase split score() left in F_MODE64/F_MODE32.
+    // Returns per instance 0, 1 = Farkas ray (in ry), 2 = improving ray (in rx).
+    std::vector<int> rayTests() {
+        const int ns = nsegs();
+        if (anyOf(fl[F_MODE64]))
+            k_ray_dir<double><<<ns, kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flagDev(F_MODE64), X.cs.p, X.rs.p,
+                                                      X.it64.view(), X.rx.p, X.ry.p, X.part_seg.p);
+        if (anyOf(fl[F_MODE32]))
+            k_ray_dir<float><<<ns, kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flagDev(F_MODE32), X.cs.p, X.rs.p,
+                                                     X.it32.view(), X.rx.p, X.ry.p, X.part_seg2.p);
         HIPCHK(hipGetLastError());
-    }
-
-    void copyD(double* dst, const double* src, size_t count) {
-        if (count) HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(double), hipMemcpyDeviceToDevice, s));
-    }
-
-    template <typename T>
-    void copyT(T* dst, const T* src, size_t count) {
-        if (count) HIPCHK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToDevice, s));
-    }
-
-    template <typename T>
-    void zeroT(T* p, int64_t count) {
-        if (count) k_fill<T><<<streamGrid(count), kThreads, 0, s>>>(count, p, T(0));
+        // squared norms (both precisions' partials) into red[16B..18B)
+        reduce({{X.part_seg.p, X.colSegBeg.p}, {X.part_seg.p, X.rowSegBegAll.p}, {X.part_seg2.p, X.colSegBeg.p},
+                {X.part_seg2.p, X.rowSegBegAll.p}},
+               false, 16);
+        // per instance pick the precision that wrote the partials
+        std::vector<double> pick(2 * size_t(B));
+        std::vector<double> flat(4 * size_t(B));
+        HIPCHK(hipMemcpyAsync(flat.data(), X.red.p + size_t(16) * B, sizeof(double) * flat.size(),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int k = 0; k < B; ++k) {
+            const bool f32 = fl[F_MODE32][k] != 0;
+            pick[k] = flat[size_t(f32 ? 2 : 0) * B + k];
+            pick[B + k] = flat[size_t(f32 ? 3 : 1) * B + k];
+        }
+        HIPCHK(hipMemcpyAsync(X.red.p + size_t(20) * B, pick.data(), sizeof(double) * pick.size(),
+                              hipMemcpyHostToDevice, s));
+        const double* nx2 = X.red.p + size_t(20) * B;
+        const double* ny2 = X.red.p + size_t(21) * B;
+        k_farkas_rows<<<X.nrs, kThreads, 0, s>>>(X.segs.p, X.ncs, flagDev(F_MODE), X.ry.p, ny2, X.rlo.p, X.rhi.p,
+                                                 X.u.p, X.part_seg2.p, X.nrs);
+        k_farkas_cols<<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.AT.val.p), flagDev(F_MODE), X.u.p, X.lo.p,
+                                                     X.hi.p, X.part_cols.p);
+        k_ray_cols<<<X.ncs, kThreads, 0, s>>>(X.segs.p, flagDev(F_MODE), X.rx.p, nx2, X.c.p, X.lo.p, X.hi.p,
+                                              X.proj.p, X.part_seg3.p, X.ncs);
+        k_ray_rows<<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.A.val.p), flagDev(F_MODE), X.proj.p, X.rlo.p,
+                                                 X.rhi.p, X.part_rows.p);
         HIPCHK(hipGetLastError());
-    }
-
-    // Ray tests (infeasibility.cpp:51-192) on the iterate's displacement since
-    // the last restart.  Returns 1 = Farkas ray (ry), 2 = improving ray (rx).
-    template <typename T>
-    int rayTests() {
-        IterBuf<T>& I = iters<T>();
-        const int p = hs.parity;
-        const int g = streamGrid(X.n + X.m);
-        k_ray_dir<T><<<g, kThreads, 0, s>>>(X.n, X.m, X.cs.p, X.rs.p, I.x[p].p, I.anx.p, I.y[p].p, I.any.p,
-                                             X.rx.p, X.ry.p, X.part_s.p);
-        HIPCHK(hipGetLastError());
-        // norms^2 into red[0] (x) and red[1] (y) for the next kernels
-        ReduceJob job{};
-        job.src[0] = X.part_s.p;
-        job.count[0] = g;
-        job.dst[0] = X.red.p + 16;
-        job.src[1] = X.part_s.p + g;
-        job.count[1] = g;
-        job.dst[1] = X.red.p + 17;
-        k_reduce<<<2, kReduceThreads, 0, s>>>(job);
-        const double* norms = X.red.p + 16;
-        const int gm = streamGrid(X.m), gn = streamGrid(X.n);
-        k_farkas_rows<<<gm, kThreads, 0, s>>>(X.m, X.ry.p, norms, X.rlo.p, X.rhi.p, X.u.p, X.part_s2.p);
-        k_farkas_cols<<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.AT.val.p), X.u.p, X.lo.p, X.hi.p,
-                                                     X.part_cols.p);
-        k_ray_cols<<<gn, kThreads, 0, s>>>(X.n, X.rx.p, norms, X.c.p, X.lo.p, X.hi.p, X.proj.p, X.part_s3.p);
-        k_ray_rows<<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.A.val.p), X.proj.p, X.rlo.p, X.rhi.p,
-                                                 X.part_rows.p);
-        HIPCHK(hipGetLastError());
-        const int nc = X.AT.nblk, nr = X.A.nblk;
-        const auto v = reduce({{X.part_s2.p, gm}, {X.part_s2.p + gm, gm}, {X.part_cols.p, nc},
-                               {X.part_cols.p + nc, nc}, {X.part_s3.p, gn}, {X.part_s3.p + gn, gn},
-                               {X.part_rows.p, nr}, {X.red.p + 16, 1}, {X.red.p + 17, 1}});
-        const double nx = std::sqrt(v[7]), ny = std::sqrt(v[8]);
-        // Dual ray: value = box infimum - support, violation from both sides.
-        if (X.m > 0 && ny > 1e-300 && std::isfinite(ny)) {
-            const double value = v[3] - v[1];
-            if (std::isfinite(value) && value > 0.0) {
-                const double viol = std::sqrt(v[2] + v[0]);
-                if (viol / value <= o.infeasibility_tolerance) return 1;
+        const int nc = X.AT.nblk;
+        const auto v = reduce({{X.part_seg2.p, X.rowSegBeg.p},
+                               {X.part_seg2.p + X.nrs, X.rowSegBeg.p},
+                               {X.part_cols.p, X.AT.ibeg.p},
+                               {X.part_cols.p + nc, X.AT.ibeg.p},
+                               {X.part_seg3.p, X.colSegBeg.p},
+                               {X.part_seg3.p + X.ncs, X.colSegBeg.p},
+                               {X.part_rows.p, X.A.ibeg.p}});
+        std::vector<int> out(B, 0);
+        for (int k = 0; k < B; ++k) {
+            if (!fl[F_MODE][k]) continue;
+            const InstInfo& in = X.info[X.dev[k]];
+            const double nx = std::sqrt(pick[k]), ny = std::sqrt(pick[B + k]);
+            if (in.m > 0 && ny > 1e-300 && std::isfinite(ny)) {
+                const double value = v[3][k] - v[1][k];
+                if (std::isfinite(value) && value > 0.0 &&
+                    std::sqrt(v[2][k] + v[0][k]) / value <= o.infeasibility_tolerance) {
+                    out[k] = 1;
+                    continue;
+                }
+            }
+            if (in.n > 0 && nx > 1e-300 && std::isfinite(nx)) {
+                const double rate = v[5][k];
+                if (std::isfinite(rate) && rate < 0.0 &&
+                    std::sqrt(v[4][k] + v[6][k]) / (-rate) <= o.infeasibility_tolerance)
+                    out[k] = 2;
             }
         }
-        if (X.n > 0 && nx > 1e-300 && std::isfinite(nx)) {
-            const double rate = v[5];
-            if (std::isfinite(rate) && rate < 0.0) {
-                const double viol = std::sqrt(v[4] + v[6]);
-                if (viol / (-rate) <= o.infeasibility_tolerance) return 2;
-            }
-        }
-        return 0;
-    }
-
-    // Squared displacement of (x, y) from the restart anchors.
-    template <typename T>
-    std::pair<double, double> displacement(const T* x, const T* y) {
-        IterBuf<T>& I = iters<T>();
-        const int g = streamGrid(X.n + X.m);
-        k_sqdist<T><<<g, kThreads, 0, s>>>(X.n, x, I.anx.p, X.m, y, I.any.p, X.part_s.p);
-        HIPCHK(hipGetLastError());
-        const auto v = reduce({{X.part_s.p, g}, {X.part_s.p + g, g}});
-        return {v[0], v[1]};
+        return out;
     }
 
     // Step controller's primal-weight update (step_controller.cpp:80-98).
-    void reweight(double dx, double dy) {
+    void reweight(int k, double dx, double dy) {
         if (dx <= 1e-16 || dy <= 1e-16 || !std::isfinite(dx) || !std::isfinite(dy)) return;
         const double target = std::clamp(dy / dx, 1e-12, 1e12);
         const double sm = std::clamp(o.primal_weight_smoothing, 0.0, 1.0);
-        setWeight(std::exp(sm * std::log(target) + (1.0 - sm) * std::log(hs.omega)));
+        setWeight(k, std::exp(sm * std::log(target) + (1.0 - sm) * std::log(hs[k].omega)));
     }
-    void setWeight(double w) {
-        if (w > 0.0 && std::isfinite(w)) hs.omega = std::clamp(w, o.minimum_primal_weight, o.maximum_primal_weight);
+    void setWeight(int k, double w) {
+        if (w > 0.0 && std::isfinite(w))
+            hs[k].omega = std::clamp(w, o.minimum_primal_weight, o.maximum_primal_weight);
+    }
+    void resetSteps(int k, double safe) {   // StepController constructor (step_controller.cpp:17-28)
+        DevState& h = hs[k];
+        h.max_safe = safe;
+        h.min_step = o.minimum_step_scale * safe;
+        h.eta = std::clamp(o.initial_step_safety * safe, o.minimum_step_scale * safe, safe);
+        h.eta_trial = h.eta;
+        h.omega = 1.0;
     }
 
-    void resetSteps(double safe) {   // StepController constructor (step_controller.cpp:17-28)
-        hs.max_safe = safe;
-        hs.min_step = o.minimum_step_scale * safe;
-        hs.eta = std::clamp(o.initial_step_safety * safe, o.minimum_step_scale * safe, safe);
-        hs.eta_trial = hs.eta;
-        hs.omega = 1.0;
-    }
-
-    // Loads a working-coordinate start (from original-coordinate xo/yo) into
-    // the precision-T iterate, refreshes A x, sets the anchors and clears the
-    // average.
     template <typename T>
-    void adoptOriginal(const double* xo, const double* yo) {
-        IterBuf<T>& I = iters<T>();
-        hs.parity = 0;
-        k_to_working<T><<<streamGrid(X.n + X.m) * 4, kThreads, 0, s>>>(X.n, X.m, X.cs.p, X.rs.p, xo, yo,
-                                                                        I.x[0].p, I.y[0].p);
+    void refresh(const int32_t* flag) {
+        k_refresh<T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<T>(work<T>().vr.p), iters<T>().view(), X.st.p, flag);
+    }
+
+    // fn(double{}, flags64) and/or fn(float{}, flags32) for the precisions
+    // whose F_AUX array has a nonzero entry.
+    template <typename Fn>
+    void phaseOp(Fn&& fn) {
+        if (anyOf(fl[F_AUX64])) fn(double{}, flagDev(F_AUX64));
+        if (anyOf(fl[F_AUX32])) fn(float{}, flagDev(F_AUX32));
+    }
+
+    // For the instances with fl[F_SEL] != 0: load the original-coordinate
+    // point (xo, yo) as the current iterate of the instance's phase (parity
+    // 0), refresh A x, set the anchors and clear the average.
+    void adopt(const double* xo, const double* yo) {
+        std::vector<int32_t>& a64 = fl[F_AUX64];
+        std::vector<int32_t>& a32 = fl[F_AUX32];
+        for (int k = 0; k < B; ++k) {
+            a64[k] = a32[k] = 0;
+            if (!fl[F_SEL][k]) continue;
+            (hs[k].phase ? a32 : a64)[k] = 1;
+            hs[k].parity = 0;
+            hs[k].avg_total = 0.0;
+        }
+        pushState();
+        pushFlags({F_AUX64, F_AUX32});
+        phaseOp([&](auto tag, const int32_t* flag) {
+            using T = decltype(tag);
+            k_to_working<T><<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.cs.p, X.rs.p, flag, xo, yo,
+                                                         iters<T>().view());
+            refresh<T>(flag);
+        });
+        for (int k = 0; k < B; ++k) {
+            a64[k] = a64[k] ? 3 : 0;
+            a32[k] = a32[k] ? 3 : 0;
+        }
+        pushFlags({F_AUX64, F_AUX32});
+        phaseOp([&](auto tag, const int32_t* flag) {
+            using T = decltype(tag);
+            k_restart<T><<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flag, iters<T>().view());
+        });
         HIPCHK(hipGetLastError());
-        refreshActivity<T>();
-        copyT(I.anx.p, I.x[0].p, X.n);
-        copyT(I.any.p, I.y[0].p, X.m);
-        zeroT(I.avx.p, X.n);
-        zeroT(I.avy.p, X.m);
-        hs.avg_total = 0.0;
     }
 
-    struct Best {
-        Metrics m;
-    } best;
+    void selectInto(double* dx, double* dy, const double* ax, const double* ay, const double* bx, const double* by) {
+        pushFlags({F_SEL});
+        k_select<<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, flagDev(F_SEL), dx, dy, ax, ay, bx, by);
+        HIPCHK(hipGetLastError());
+    }
 
-    template <typename T>
-    bool mainLoop(int& status, std::string& msg, int& restarts);
-    template <typename T>
-    void polish(int64_t& pIts, int64_t& pTrials, Metrics& pBest, bool& improved);
+    void initStart(const double* const* warmX, const double* const* warmY);
+    void mainLoop();
+    void checkMain(const std::vector<int>& atCheck);
+    void polishStage();
+    void collect(pdlpx_summary* const* sums, double* const* xOut, double* const* yOut, double* const* rayOut);
 };
 
 template <>
-Working<double>& Run::work<double>() { return X.w64; }
+const Working<double>& Run::work<double>() const { return X.w64; }
 template <>
-Working<float>& Run::work<float>() { return X.w32; }
+const Working<float>& Run::work<float>() const { return X.w32; }
 template <>
-IterBuf<double>& Run::iters<double>() { return X.it64; }
+const IterBuf<double>& Run::iters<double>() const { return X.it64; }
 template <>
-IterBuf<float>& Run::iters<float>() { return X.it32; }
+const IterBuf<float>& Run::iters<float>() const { return X.it32; }
 
 // Set-up on the device: Ruiz (scaling.cpp:55-136), the scaled problem, and
 // Pock-Chambolle (preconditioner.cpp:31-80).
 void Run::setup() {
     const int m = X.m, n = X.n;
     const int bt = 256;
+    fl.assign(F_COUNT, std::vector<int32_t>(B, 0));
+    hs.assign(B, DevState{});
+    I.assign(B, Inst{});
+    rays.assign(B, {});
     k_fill<double><<<gridFor(m), kThreads, 0, s>>>(m, X.rs.p, 1.0);
     k_fill<double><<<gridFor(n), kThreads, 0, s>>>(n, X.cs.p, 1.0);
-    scaled = o.use_ruiz_scaling && o.ruiz_iterations > 0 && X.nnz > 0;
+    const bool scaled = o.use_ruiz_scaling && o.ruiz_iterations > 0 && X.nnz > 0;
     if (scaled) {
         for (int pass = 0; pass < o.ruiz_iterations; ++pass) {
             k_ruiz_max<<<gridFor(m, bt), bt, 0, s>>>(m, X.A.ptr.p, X.A.idx.p, X.A.val.p, X.rs.p, X.cs.p, true,
                                                      X.rmax.p);
-            k_ruiz_max<<<gridFor(n, bt), bt, 0, s>>>(n, X.AT.ptr.p, X.AT.idx.p, X.AT.val.p, X.cs.p, X.rs.p,
-                                                     false, X.cmax.p);
+            k_ruiz_max<<<gridFor(n, bt), bt, 0, s>>>(n, X.AT.ptr.p, X.AT.idx.p, X.AT.val.p, X.cs.p, X.rs.p, false,
+                                                     X.cmax.p);
             k_ruiz_update<<<gridFor(m, bt), bt, 0, s>>>(m, X.rs.p, X.rmax.p);
             k_ruiz_update<<<gridFor(n, bt), bt, 0, s>>>(n, X.cs.p, X.cmax.p);
         }
     }
-    Working<double>& w = X.w64;
+    const Working<double>& w = X.w64;
     const bool f32 = o.precision == PDLPX_FP32;
     float* v32r = f32 ? X.w32.vr.p : nullptr;
     float* v32c = f32 ? X.w32.vc.p : nullptr;
@@ -593,479 +715,700 @@ void Run::setup() {
     k_scale_cols<<<gridFor(n, bt), bt, 0, s>>>(n, X.cs.p, X.c.p, X.lo.p, X.hi.p, w.c.p, w.lo.p, w.hi.p);
     k_scale_rows<<<gridFor(m, bt), bt, 0, s>>>(m, X.rs.p, X.rlo.p, X.rhi.p, w.rlo.p, w.rhi.p);
     if (o.use_diagonal_preconditioning) {
-        k_abs_sums<<<gridFor(n, bt), bt, 0, s>>>(n, X.AT.ptr.p, w.vc.p, w.tau.p, w.tau_inv.p);
-        k_abs_sums<<<gridFor(m, bt), bt, 0, s>>>(m, X.A.ptr.p, w.vr.p, w.sig.p, w.sig_inv.p);
+        k_abs_sums<<<gridFor(n, bt), bt, 0, s>>>(n, X.AT.ptr.p, w.vc.p, w.tau.p);
+        k_abs_sums<<<gridFor(m, bt), bt, 0, s>>>(m, X.A.ptr.p, w.vr.p, w.sig.p);
     } else {
-        for (auto* v : {&w.tau, &w.tau_inv}) k_fill<double><<<gridFor(n), kThreads, 0, s>>>(n, v->p, 1.0);
-        for (auto* v : {&w.sig, &w.sig_inv}) k_fill<double><<<gridFor(m), kThreads, 0, s>>>(m, v->p, 1.0);
+        k_fill<double><<<gridFor(n), kThreads, 0, s>>>(n, w.tau.p, 1.0);
+        k_fill<double><<<gridFor(m), kThreads, 0, s>>>(m, w.sig.p, 1.0);
     }
     HIPCHK(hipGetLastError());
     if (f32) {
-        Working<float>& h = X.w32;
-        const int g = streamGrid(std::max(n, m)) * 4;
-        for (auto pr : {std::make_pair(&h.c, &w.c), std::make_pair(&h.lo, &w.lo), std::make_pair(&h.hi, &w.hi),
-                        std::make_pair(&h.tau, &w.tau), std::make_pair(&h.tau_inv, &w.tau_inv)})
-            k_convert<float, double><<<g, kThreads, 0, s>>>(n, pr.second->p, pr.first->p);
-        for (auto pr : {std::make_pair(&h.rlo, &w.rlo), std::make_pair(&h.rhi, &w.rhi),
-                        std::make_pair(&h.sig, &w.sig), std::make_pair(&h.sig_inv, &w.sig_inv)})
-            k_convert<float, double><<<g, kThreads, 0, s>>>(m, pr.second->p, pr.first->p);
+        const Working<float>& h = X.w32;
+        const int g = std::min(2048, gridFor(std::max(n, m)));
+        for (auto pr : {std::make_pair(h.c.p, w.c.p), std::make_pair(h.lo.p, w.lo.p),
+                        std::make_pair(h.hi.p, w.hi.p), std::make_pair(h.tau.p, w.tau.p)})
+            k_convert<float, double><<<g, kThreads, 0, s>>>(n, pr.second, pr.first);
+        for (auto pr : {std::make_pair(h.rlo.p, w.rlo.p), std::make_pair(h.rhi.p, w.rhi.p),
+                        std::make_pair(h.sig.p, w.sig.p)})
+            k_convert<float, double><<<g, kThreads, 0, s>>>(m, pr.second, pr.first);
         HIPCHK(hipGetLastError());
     }
-    maxSafe = 0.99;
+    maxSafe.assign(B, 0.99);
     if (!o.use_diagonal_preconditioning) {   // pdlp_solver.cpp:189-202
-        const double norm = powerIteration();
-        // powerIteration() stores convergence in hs.fail temporarily
-        const bool conv = hs.fail == 0;
-        hs.fail = 0;
-        maxSafe = norm > 0.0 ? 0.99 / (norm * (conv ? 1.02 : 1.25)) : 1.0;
+        std::vector<char> conv;
+        const std::vector<double> norm = powerIteration(conv);
+        for (int k = 0; k < B; ++k) maxSafe[k] = norm[k] > 0.0 ? 0.99 / (norm[k] * (conv[k] ? 1.02 : 1.25)) : 1.0;
     }
 }
 
-// Power iteration on A^T A of the working matrix (preconditioner.cpp:82-140).
-double Run::powerIteration() {
-    hs.fail = 0;
-    if (X.m == 0 || X.n == 0 || X.nnz == 0) return 0.0;
-    std::mt19937 gen(1);
-    std::uniform_real_distribution<double> dist(-1.0, 1.0);
+// Power iteration on A^T A of each instance's working matrix
+// (preconditioner.cpp:82-140).  All instances iterate together; each keeps the
+// estimate of the iteration where it stopped.
+std::vector<double> Run::powerIteration(std::vector<char>& converged) {
+    std::vector<double> est(B, 0.0), prev(B, 0.0);
+    std::vector<char> stopped(B, 0);
+    converged.assign(B, 1);
     std::vector<double> x(static_cast<size_t>(X.n));
-    for (double& e : x) e = dist(gen);
-    double sq = 0.0;
-    for (double e : x) sq += e * e;
-    double norm = std::sqrt(sq);
-    if (norm > 0.0) {
-        const double inv = 1.0 / norm;
-        for (double& e : x) e *= inv;
-    } else {
-        x.assign(x.size(), 1.0 / std::sqrt(static_cast<double>(x.size())));
+    for (int k = 0; k < B; ++k) {
+        const InstInfo& in = X.info[X.dev[k]];
+        if (in.m == 0 || in.n == 0 || in.nnz == 0) {
+            stopped[k] = 1;
+            continue;
+        }
+        std::mt19937 gen(1);
+        std::uniform_real_distribution<double> dist(-1.0, 1.0);
+        double* xs = x.data() + X.colOff[k];
+        for (int j = 0; j < in.n; ++j) xs[j] = dist(gen);
+        double sq = 0.0;
+        for (int j = 0; j < in.n; ++j) sq += xs[j] * xs[j];
+        const double norm = std::sqrt(sq);
+        if (norm > 0.0) {
+            const double inv = 1.0 / norm;
+            for (int j = 0; j < in.n; ++j) xs[j] *= inv;
+        } else {
+            for (int j = 0; j < in.n; ++j) xs[j] = 1.0 / std::sqrt(static_cast<double>(in.n));
+        }
+        converged[k] = 0;
     }
     double* dx = X.xo0.p;
     double* dy = X.yo0.p;
     double* dz = X.xo1.p;
-    HIPCHK(hipMemcpyAsync(dx, x.data(), x.size() * 8, hipMemcpyHostToDevice, s));
+    if (X.n) HIPCHK(hipMemcpyAsync(dx, x.data(), x.size() * 8, hipMemcpyHostToDevice, s));
     const int budget = std::max(o.power_iteration_count, 1);
-    double prev = 0.0, est = 0.0;
-    bool hit = false;
-    const int gn = streamGrid(X.n);
+    std::vector<double> factor(B, 1.0);
     for (int it = 0; it < budget; ++it) {
+        if (std::all_of(stopped.begin(), stopped.end(), [](char c) { return c != 0; })) break;
         k_spmv<double><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.w64.vr.p), dx, dy);
         k_spmv<double><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.w64.vc.p), dy, dz);
-        k_dot<<<gn, kThreads, 0, s>>>(X.n, dx, dz, X.part_s.p);
-        k_dot<<<gn, kThreads, 0, s>>>(X.n, dz, dz, X.part_s.p + gn);
+        k_dot_seg<<<X.ncs, kThreads, 0, s>>>(X.segs.p, dx, dz, X.part_seg.p);
+        k_dot_seg<<<X.ncs, kThreads, 0, s>>>(X.segs.p, dz, dz, X.part_seg2.p);
         HIPCHK(hipGetLastError());
-        const auto v = reduce({{X.part_s.p, gn}, {X.part_s.p + gn, gn}});
-        est = std::sqrt(std::max(v[0], 0.0));
-        const double zn = std::sqrt(v[1]);
-        if (!(zn > 0.0)) {
-            hs.fail = 0;
-            return est;
+        const auto v = reduce({{X.part_seg.p, X.colSegBeg.p}, {X.part_seg2.p, X.colSegBeg.p}});
+        for (int k = 0; k < B; ++k) {
+            factor[k] = 1.0;
+            if (stopped[k]) continue;
+            est[k] = std::sqrt(std::max(v[0][k], 0.0));
+            const double zn = std::sqrt(v[1][k]);
+            if (!(zn > 0.0)) {
+                stopped[k] = 1;
+                converged[k] = 1;
+                continue;
+            }
+            factor[k] = 1.0 / zn;
+            if (it > 0 && est[k] > 0.0 && std::abs(est[k] - prev[k]) <= 1e-8 * est[k]) {
+                stopped[k] = 1;
+                converged[k] = 1;
+            }
+            prev[k] = est[k];
         }
-        k_scale_vec<<<gridFor(X.n), kThreads, 0, s>>>(X.n, dz, 1.0 / zn);
+        HIPCHK(hipMemcpyAsync(X.red.p, factor.data(), sizeof(double) * B, hipMemcpyHostToDevice, s));
+        k_scale_seg<<<X.ncs, kThreads, 0, s>>>(X.segs.p, dz, X.red.p);
+        HIPCHK(hipGetLastError());
         std::swap(dx, dz);
-        if (it > 0 && est > 0.0 && std::abs(est - prev) <= 1e-8 * est) {
-            hit = true;
-            break;
-        }
-        prev = est;
     }
-    hs.fail = hit ? 0 : 1;
     return est;
 }
 
-// The PDLP outer loop (pdlp_solver.cpp:277-477) in precision T.  Returns true
-// when the solve is finished (status/msg set); false when it ran out of
-// iterations or time (status/msg hold the limit) or, in the FP32 phase, when it
-// decided to hand over to FP64.
-template <typename T>
-bool Run::mainLoop(int& status, std::string& msg, int& restarts) {
-    IterBuf<T>& I = iters<T>();
-    const bool f32phase = sizeof(T) == 4;
-    const int freq = o.termination_check_frequency;
-    double baseline = best.m.kkt;   // restart baseline
-    double prevScore = kInf;        // RestartController::previousCandidateScore_
-    int64_t sinceRestart = 0;
-    int stall = 0;
-    double lastBest = best.m.kkt;
-    while (hs.iteration < o.iteration_limit) {
-        if (o.time_limit_seconds > 0.0 && since() >= o.time_limit_seconds) {
-            status = PDLPX_TIME_LIMIT;
-            msg = "Time limit reached";
-            return false;
-        }
-        const int64_t target = std::min<int64_t>((hs.iteration / freq + 1) * freq, o.iteration_limit);
-        const int64_t before = hs.iteration;
-        runTrials<T>(target);
-        sinceRestart += hs.iteration - before;
-        if (f32phase) fp32Its += hs.iteration - before;
-        if (hs.fail == 1) {
-            status = PDLPX_NUMERICAL_FAILURE;
-            msg = "PDHG generated a non-finite iterate";
-            return true;
-        }
-        if (hs.fail == 2) {
-            status = PDLPX_NUMERICAL_FAILURE;
-            msg = "Adaptive linesearch failed to accept a step";
-            return true;
-        }
-        if (hs.iteration % freq != 0) continue;   // stopped at the iteration limit
-
-        // ---- termination check (pdlp_solver.cpp:354-435) ----
-        const int p = hs.parity;
-        const bool haveAvg = o.use_averaging && hs.avg_total != 0.0;
-        toOriginal<T>(I.x[p].p, I.y[p].p, X.xo0.p, X.yo0.p);
-        if (haveAvg) toOriginal<T>(I.avx.p, I.avy.p, X.xo1.p, X.yo1.p);
-        else {
-            copyD(X.xo1.p, X.xo0.p, X.n);
-            copyD(X.yo1.p, X.yo0.p, X.m);
-        }
-        Metrics cur, avg;
-        scorePair(cur, avg);
-        if (!haveAvg) avg = cur;
-        if (cur.kkt < best.m.kkt) {
-            copyD(X.bestx.p, X.xo0.p, X.n);
-            copyD(X.besty.p, X.yo0.p, X.m);
-            best.m = cur;
-        }
-        if (haveAvg && avg.kkt < best.m.kkt) {
-            copyD(X.bestx.p, X.xo1.p, X.n);
-            copyD(X.besty.p, X.yo1.p, X.m);
-            best.m = avg;
-        }
-        if (o.detect_infeasibility) {
-            const int ray = rayTests<T>();
-            if (ray == 1) {
-                status = PDLPX_INFEASIBLE;
-                msg = "Primal infeasible: Farkas ray found";
-                return true;
-            }
-            if (ray == 2) {
-                status = PDLPX_UNBOUNDED;
-                msg = "Unbounded: improving ray found";
-                return true;
-            }
-        }
-        if (optimal(best.m)) {
-            status = PDLPX_OPTIMAL;
-            msg = "Primal, dual, and gap tolerances satisfied";
-            return true;
-        }
-        if (f32phase) {   // FP32 -> FP64 hand-off
-            const double tol = std::max(o.primal_tolerance, std::max(o.dual_tolerance, o.gap_tolerance));
-            stall = best.m.kkt < lastBest ? 0 : stall + 1;
-            lastBest = std::min(lastBest, best.m.kkt);
-            if (best.m.kkt <= o.fp32_handoff_factor * tol ||
-                (o.fp32_stall_checks > 0 && stall >= o.fp32_stall_checks))
-                return false;
-        }
-
-        // ---- restart (restart_controller.cpp:8-63, pdlp_solver.cpp:437-476) ----
-        int pick = 0;   // 1 current, 2 average
-        double score = kInf;
-        if (o.use_restarts && sinceRestart >= o.minimum_restart_iterations) {
-            const bool af = std::isfinite(avg.kkt), cf = std::isfinite(cur.kkt);
-            if (af || cf) {
-                int cand = 2;
-                score = avg.kkt;
-                if (!af || (cf && cur.kkt < avg.kkt)) {
-                    cand = 1;
-                    score = cur.kkt;
-                }
-                const double prev = prevScore;
-                prevScore = score;
-                const bool bf = std::isfinite(baseline);
-                const auto art = static_cast<int64_t>(o.restart_artificial_fraction * static_cast<double>(hs.iteration));
-                if (bf && score <= o.restart_sufficient_decay * baseline) pick = cand;
-                else if (bf && score <= o.restart_necessary_decay * baseline && std::isfinite(prev) && score > prev)
-                    pick = cand;
-                else if (sinceRestart >= std::max<int64_t>(art, 1) || sinceRestart >= o.maximum_restart_iterations)
-                    pick = cand;
-            }
-        }
-        if (pick != 0) {
-            const bool fromAvg = pick == 2 && haveAvg;
-            const T* sx = fromAvg ? I.avx.p : I.x[p].p;
-            const T* sy = fromAvg ? I.avy.p : I.y[p].p;
-            const auto d = displacement<T>(sx, sy);
-            reweight(std::sqrt(d.first), std::sqrt(d.second));
-            if (fromAvg) {
-                copyT(I.x[p].p, I.avx.p, X.n);
-                copyT(I.y[p].p, I.avy.p, X.m);
-                refreshActivity<T>();
-            }
-            copyT(I.anx.p, I.x[p].p, X.n);
-            copyT(I.any.p, I.y[p].p, X.m);
-            baseline = score;
-            zeroT(I.avx.p, X.n);
-            zeroT(I.avy.p, X.m);
-            hs.avg_total = 0.0;
-            prevScore = kInf;
-            sinceRestart = 0;
-            ++restarts;
+void Run::initStart(const double* const* warmX, const double* const* warmY) {
+    const int n = X.n, m = X.m;
+    for (int k = 0; k < B; ++k) {
+        const int u = X.dev[k];
+        fl[F_WX][k] = warmX && warmX[u] ? 1 : 0;
+        fl[F_WY][k] = warmY && warmY[u] ? 1 : 0;
+        if (fl[F_WX][k] && X.info[u].n)
+            HIPCHK(hipMemcpyAsync(X.warmx.p + X.colOff[k], warmX[u], sizeof(double) * X.info[u].n,
+                                  hipMemcpyHostToDevice, s));
+        if (fl[F_WY][k] && X.info[u].m)
+            HIPCHK(hipMemcpyAsync(X.warmy.p + X.rowOff[k], warmY[u], sizeof(double) * X.info[u].m,
+                                  hipMemcpyHostToDevice, s));
+    }
+    pushFlags({F_WX, F_WY});
+    const IterBuf<double>& D = X.it64;
+    k_start<<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.w64.lo.p, X.w64.hi.p, X.w64.rlo.p, X.w64.rhi.p, X.cs.p,
+                                         X.rs.p, flagDev(F_WX), flagDev(F_WY), X.warmx.p, X.warmy.p, D.x[0].p,
+                                         D.y[0].p);
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < B; ++k) {
+        DevState& h = hs[k];
+        std::memset(&h, 0, sizeof(h));
+        resetSteps(k, maxSafe[k]);
+        setWeight(k, o.initial_primal_weight);
+        h.adaptive = o.use_adaptive_linesearch ? 1 : 0;
+        h.averaging = o.use_averaging ? 1 : 0;
+        h.max_trials = std::max(o.maximum_step_trials, 1);
+        h.done = 1;
+    }
+    pushState();
+    // score the start (pdlp_solver.cpp:239-264)
+    std::fill(fl[F_MODE].begin(), fl[F_MODE].end(), 1);
+    std::vector<Metrics> cur, avg;
+    score(cur, avg);
+    std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 1);
+    selectInto(X.bestx.p, X.besty.p, X.xo0.p, X.yo0.p, X.xo1.p, X.yo1.p);
+    for (int k = 0; k < B; ++k) {
+        I[k].best = cur[k];
+        if (optimal(cur[k])) {
+            I[k].mainDone = I[k].finished = true;
+            I[k].status = PDLPX_OPTIMAL;
+            I[k].msg = "Initial point satisfies the requested tolerances";
+            I[k].finishTime = since();
+        } else {
+            I[k].diag = true;
+            I[k].baseline = cur[k].kkt;
+            I[k].lastBest = cur[k].kkt;
         }
     }
-    status = PDLPX_ITERATION_LIMIT;
-    msg = "Iteration limit reached";
-    return false;
+    // the running instances' phase, start iterate, A x and anchors
+    if (o.precision == PDLPX_FP32) {
+        const IterBuf<float>& F = X.it32;
+        const int g = std::min(2048, gridFor(std::max(n, m)));
+        k_convert<float, double><<<g, kThreads, 0, s>>>(n, D.x[0].p, F.x[0].p);
+        k_convert<float, double><<<g, kThreads, 0, s>>>(m, D.y[0].p, F.y[0].p);
+        HIPCHK(hipGetLastError());
+        for (int k = 0; k < B; ++k)
+            if (!I[k].finished) hs[k].phase = 1;
+    }
+    for (int k = 0; k < B; ++k) {
+        fl[F_AUX64][k] = (!I[k].finished && hs[k].phase == 0) ? 3 : 0;
+        fl[F_AUX32][k] = (!I[k].finished && hs[k].phase == 1) ? 3 : 0;
+    }
+    pushState();
+    pushFlags({F_AUX64, F_AUX32});
+    phaseOp([&](auto tag, const int32_t* flag) {
+        using T = decltype(tag);
+        refresh<T>(flag);
+        k_restart<T><<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flag, iters<T>().view());
+    });
+    HIPCHK(hipGetLastError());
 }
 
-// FeasibilityPolisher::polish (feasibility_polishing.cpp:13-145): a fresh step
-// controller from the best point at half the last step, averaging, checks at
-// the usual cadence, keep the best complete KKT score.
-template <typename T>
-void Run::polish(int64_t& pIts, int64_t& pTrials, Metrics& pBest, bool& improved) {
-    IterBuf<T>& I = iters<T>();
-    pIts = 0;
-    pTrials = 0;
-    improved = false;
-    if (!o.use_feasibility_polishing || o.polishing_iterations <= 0) return;
+// The PDLP outer loop (pdlp_solver.cpp:277-477) for every running instance.
+void Run::mainLoop() {
+    const int freq = o.termination_check_frequency;
+    std::vector<int64_t> before(B);
+    while (true) {
+        bool any = false;
+        for (int k = 0; k < B; ++k) {
+            DevState& h = hs[k];
+            h.done = 1;
+            if (I[k].mainDone) continue;
+            if (h.iteration >= o.iteration_limit) {
+                I[k].mainDone = true;   // the limit status is the default
+                continue;
+            }
+            h.target = std::min<int64_t>((h.iteration / freq + 1) * freq, o.iteration_limit);
+            h.done = 0;
+            any = true;
+        }
+        if (!any) break;
+        if (o.time_limit_seconds > 0.0 && since() >= o.time_limit_seconds) {
+            for (int k = 0; k < B; ++k)
+                if (!I[k].mainDone) {
+                    I[k].mainDone = true;
+                    I[k].status = PDLPX_TIME_LIMIT;
+                    I[k].msg = "Time limit reached";
+                    hs[k].done = 1;
+                }
+            break;
+        }
+        for (int k = 0; k < B; ++k) before[k] = hs[k].iteration;
+        runTrials();
+        std::vector<int> atCheck;
+        for (int k = 0; k < B; ++k) {
+            if (I[k].mainDone) continue;
+            const DevState& h = hs[k];
+            I[k].sinceRestart += h.iteration - before[k];
+            if (h.phase) I[k].fp32Its += h.iteration - before[k];
+            if (h.fail) {
+                I[k].mainDone = I[k].finished = true;
+                I[k].status = PDLPX_NUMERICAL_FAILURE;
+                I[k].msg = h.fail == 1 ? "PDHG generated a non-finite iterate"
+                                       : "Adaptive linesearch failed to accept a step";
+                I[k].finishTime = since();
+                continue;
+            }
+            if (h.iteration % freq == 0) atCheck.push_back(k);
+        }
+        if (!atCheck.empty()) checkMain(atCheck);
+    }
+}
+
+// Termination check, ray tests, FP32 hand-off and restart for the instances in
+// `atCheck` (pdlp_solver.cpp:354-476).
+void Run::checkMain(const std::vector<int>& atCheck) {
+    std::fill(fl[F_MODE].begin(), fl[F_MODE].end(), 0);
+    for (int k : atCheck) fl[F_MODE][k] = (o.use_averaging && hs[k].avg_total != 0.0) ? 2 : 1;
+    std::vector<Metrics> cur, avg;
+    score(cur, avg);
+    std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
+    for (int k : atCheck) {   // best candidate: current, then average
+        if (fl[F_MODE][k] == 1) avg[k] = cur[k];
+        if (cur[k].kkt < I[k].best.kkt) {
+            fl[F_SEL][k] = 1;
+            I[k].best = cur[k];
+        }
+        if (fl[F_MODE][k] == 2 && avg[k].kkt < I[k].best.kkt) {
+            fl[F_SEL][k] = 2;
+            I[k].best = avg[k];
+        }
+    }
+    selectInto(X.bestx.p, X.besty.p, X.xo0.p, X.yo0.p, X.xo1.p, X.yo1.p);
+    if (o.detect_infeasibility) {
+        const std::vector<int> ray = rayTests();
+        bool anyRay = false;
+        for (int k : atCheck) {
+            if (!ray[k]) continue;
+            anyRay = true;
+            Inst& in = I[k];
+            in.mainDone = in.finished = true;
+            in.status = ray[k] == 1 ? PDLPX_INFEASIBLE : PDLPX_UNBOUNDED;
+            in.msg = ray[k] == 1 ? "Primal infeasible: Farkas ray found" : "Unbounded: improving ray found";
+            in.rayKind = in.status;
+            in.finishTime = since();
+            const InstInfo& ii = X.info[X.dev[k]];
+            std::vector<double>& dst = rays[k];
+            if (ray[k] == 1) {
+                dst.resize(ii.m);
+                if (ii.m)
+                    HIPCHK(hipMemcpyAsync(dst.data(), X.ry.p + X.rowOff[k], 8 * size_t(ii.m), hipMemcpyDeviceToHost, s));
+            } else {
+                dst.resize(ii.n);
+                if (ii.n)
+                    HIPCHK(hipMemcpyAsync(dst.data(), X.rx.p + X.colOff[k], 8 * size_t(ii.n), hipMemcpyDeviceToHost, s));
+            }
+        }
+        if (anyRay) HIPCHK(hipStreamSynchronize(s));
+    }
+    std::vector<char> handoff(B, 0);
+    for (int k : atCheck) {
+        if (I[k].finished) continue;
+        if (optimal(I[k].best)) {
+            I[k].mainDone = I[k].finished = true;
+            I[k].status = PDLPX_OPTIMAL;
+            I[k].msg = "Primal, dual, and gap tolerances satisfied";
+            I[k].finishTime = since();
+            continue;
+        }
+        if (hs[k].phase == 1) {   // FP32 -> FP64 hand-off test
+            const double tol = std::max(o.primal_tolerance, std::max(o.dual_tolerance, o.gap_tolerance));
+            I[k].stall = I[k].best.kkt < I[k].lastBest ? 0 : I[k].stall + 1;
+            I[k].lastBest = std::min(I[k].lastBest, I[k].best.kkt);
+            if (I[k].best.kkt <= o.fp32_handoff_factor * tol ||
+                (o.fp32_stall_checks > 0 && I[k].stall >= o.fp32_stall_checks))
+                handoff[k] = 1;
+        }
+    }
+    // ---- restarts (restart_controller.cpp:8-63, pdlp_solver.cpp:437-476) ----
+    std::vector<int32_t> pick(B, 0);
+    std::vector<double> pickScore(B, kInf);
+    for (int k : atCheck) {
+        if (I[k].finished || handoff[k]) continue;
+        Inst& in = I[k];
+        if (!o.use_restarts || in.sinceRestart < o.minimum_restart_iterations) continue;
+        const bool af = std::isfinite(avg[k].kkt), cf = std::isfinite(cur[k].kkt);
+        if (!af && !cf) continue;
+        int cand = 2;
+        double score = avg[k].kkt;
+        if (!af || (cf && cur[k].kkt < avg[k].kkt)) {
+            cand = 1;
+            score = cur[k].kkt;
+        }
+        const double prev = in.prevScore;
+        in.prevScore = score;
+        const bool bf = std::isfinite(in.baseline);
+        const auto art = static_cast<int64_t>(o.restart_artificial_fraction * static_cast<double>(hs[k].iteration));
+        int p = 0;
+        if (bf && score <= o.restart_sufficient_decay * in.baseline) p = cand;
+        else if (bf && score <= o.restart_necessary_decay * in.baseline && std::isfinite(prev) && score > prev)
+            p = cand;
+        else if (in.sinceRestart >= std::max<int64_t>(art, 1) || in.sinceRestart >= o.maximum_restart_iterations)
+            p = cand;
+        if (p == 2 && fl[F_MODE][k] != 2) p = 1;   // adopting the average needs one
+        pick[k] = p;
+        pickScore[k] = score;
+    }
+    if (anyOf(pick)) {
+        for (int k = 0; k < B; ++k) {
+            fl[F_AUX64][k] = hs[k].phase == 0 ? pick[k] : 0;
+            fl[F_AUX32][k] = hs[k].phase == 1 ? pick[k] : 0;
+        }
+        pushFlags({F_AUX64, F_AUX32});
+        const int ns = nsegs();
+        phaseOp([&](auto tag, const int32_t* flag) {
+            using T = decltype(tag);
+            k_sqdist<T><<<ns, kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flag, iters<T>().view(),
+                                                sizeof(T) == 8 ? X.part_seg.p : X.part_seg3.p);
+        });
+        HIPCHK(hipGetLastError());
+        const auto d = reduce({{X.part_seg.p, X.colSegBeg.p}, {X.part_seg.p, X.rowSegBegAll.p},
+                               {X.part_seg3.p, X.colSegBeg.p}, {X.part_seg3.p, X.rowSegBegAll.p}});
+        for (int k = 0; k < B; ++k) {
+            if (!pick[k]) continue;
+            const int q = hs[k].phase ? 2 : 0;
+            reweight(k, std::sqrt(d[q][k]), std::sqrt(d[q + 1][k]));
+        }
+        // adopt the average where chosen, then anchors = current, average = 0
+        phaseOp([&](auto tag, const int32_t* flag) {
+            using T = decltype(tag);
+            k_restart<T><<<ns, kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flag, iters<T>().view());
+        });
+        for (int k = 0; k < B; ++k) {
+            fl[F_AUX64][k] = fl[F_AUX64][k] == 2 ? 1 : 0;
+            fl[F_AUX32][k] = fl[F_AUX32][k] == 2 ? 1 : 0;
+        }
+        pushFlags({F_AUX64, F_AUX32});
+        phaseOp([&](auto tag, const int32_t* flag) {
+            using T = decltype(tag);
+            refresh<T>(flag);
+        });
+        HIPCHK(hipGetLastError());
+        for (int k = 0; k < B; ++k) {
+            if (!pick[k]) continue;
+            I[k].baseline = pickScore[k];
+            hs[k].avg_total = 0.0;
+            I[k].prevScore = kInf;
+            I[k].sinceRestart = 0;
+            ++I[k].restarts;
+        }
+    }
+    // ---- hand over to FP64 from the best point, as after a restart ----
+    if (std::any_of(handoff.begin(), handoff.end(), [](char c) { return c != 0; })) {
+        std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
+        for (int k = 0; k < B; ++k) {
+            if (!handoff[k]) continue;
+            fl[F_SEL][k] = 1;
+            hs[k].phase = 0;
+            I[k].baseline = I[k].best.kkt;
+            I[k].prevScore = kInf;
+            I[k].sinceRestart = 0;
+        }
+        adopt(X.bestx.p, X.besty.p);
+    }
+}
+
+// FeasibilityPolisher::polish (feasibility_polishing.cpp:13-145) for the
+// instances that left the main loop on a limit: a fresh step controller from
+// the best point at half the last step, averaging, checks at the usual
+// cadence, keep the best complete KKT score.
+void Run::polishStage() {
+    for (int k = 0; k < B; ++k) {
+        Inst& in = I[k];
+        in.mainIts = hs[k].iteration;
+        in.mainTrials = hs[k].trials;
+        in.mainEta = hs[k].eta;
+        in.mainOmega = hs[k].omega;
+        in.mainSafe = hs[k].max_safe;
+    }
+    if (!o.use_feasibility_polishing) return;
     double remaining = 0.0;
-    if (o.time_limit_seconds > 0.0) remaining = o.time_limit_seconds - since();
-    const double lastEta = hs.eta;
-    const double omega = hs.omega;
-    // best (original) -> working start
-    adoptOriginal<T>(X.bestx.p, X.besty.p);
-    resetSteps(hs.max_safe);
-    setWeight(omega);
-    hs.eta_trial = 0.5 * lastEta;
-    hs.iteration = 0;
-    hs.attempts = 0;
-    hs.fail = 0;
-    const int64_t trials0 = hs.trials;
+    if (o.time_limit_seconds > 0.0) {
+        remaining = o.time_limit_seconds - since();
+        if (remaining <= 0.0) return;
+    }
+    std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
+    bool any = false;
+    for (int k = 0; k < B; ++k) {
+        Inst& in = I[k];
+        if (in.finished || X.info[X.dev[k]].n == 0) continue;
+        in.polishing = true;
+        in.pBest = in.best;
+        if (o.polishing_iterations <= 0) continue;
+        fl[F_SEL][k] = 1;
+        any = true;
+    }
+    if (!any) return;
+    for (int k = 0; k < B; ++k) {
+        if (!fl[F_SEL][k]) continue;
+        const double lastEta = hs[k].eta, omega = hs[k].omega;
+        resetSteps(k, hs[k].max_safe);
+        setWeight(k, omega);
+        hs[k].eta_trial = 0.5 * lastEta;
+        hs[k].iteration = 0;
+        hs[k].attempts = 0;
+        hs[k].fail = 0;
+        I[k].trials0 = hs[k].trials;
+    }
+    adopt(X.bestx.p, X.besty.p);
+    std::vector<char> active(B, 0);
+    for (int k = 0; k < B; ++k) active[k] = fl[F_SEL][k] != 0;
     const auto pt0 = Clock::now();
     const int freq = std::max(o.termination_check_frequency, 1);
-    while (hs.iteration < o.polishing_iterations) {
-        const int64_t target = std::min<int64_t>((hs.iteration / freq + 1) * freq, o.polishing_iterations);
-        runTrials<T>(target);
-        pIts = hs.iteration;
-        pTrials = hs.trials - trials0;
-        if (hs.fail) break;
-        if (remaining > 0.0 &&
-            std::chrono::duration<double>(Clock::now() - pt0).count() >= remaining)
-            break;
-        const int p = hs.parity;
-        const bool haveAvg = o.use_averaging && hs.avg_total != 0.0;
-        toOriginal<T>(I.x[p].p, I.y[p].p, X.xo0.p, X.yo0.p);
-        if (haveAvg) toOriginal<T>(I.avx.p, I.avy.p, X.xo1.p, X.yo1.p);
-        else {
-            copyD(X.xo1.p, X.xo0.p, X.n);
-            copyD(X.yo1.p, X.yo0.p, X.m);
+    while (true) {
+        bool anyActive = false;
+        for (int k = 0; k < B; ++k) {
+            hs[k].done = 1;
+            if (!active[k]) continue;
+            if (hs[k].iteration >= o.polishing_iterations) {
+                active[k] = 0;
+                continue;
+            }
+            hs[k].target = std::min<int64_t>((hs[k].iteration / freq + 1) * freq, o.polishing_iterations);
+            hs[k].done = 0;
+            anyActive = true;
         }
-        Metrics cur, avg;
-        scorePair(cur, avg);
-        if (cur.kkt < pBest.kkt) {
-            copyD(X.proj.p, X.xo0.p, X.n);   // polish-best kept in proj/u
-            copyD(X.u.p, X.yo0.p, X.m);
-            pBest = cur;
-            improved = true;
+        if (!anyActive) break;
+        runTrials();
+        std::fill(fl[F_MODE].begin(), fl[F_MODE].end(), 0);
+        bool anyCheck = false;
+        const bool expired =
+            remaining > 0.0 && std::chrono::duration<double>(Clock::now() - pt0).count() >= remaining;
+        for (int k = 0; k < B; ++k) {
+            if (!active[k]) continue;
+            I[k].pIts = hs[k].iteration;
+            I[k].pTrials = hs[k].trials - I[k].trials0;
+            if (hs[k].fail || expired) {
+                active[k] = 0;
+                continue;
+            }
+            fl[F_MODE][k] = (o.use_averaging && hs[k].avg_total != 0.0) ? 2 : 1;
+            anyCheck = true;
         }
-        if (haveAvg && avg.kkt < pBest.kkt) {
-            copyD(X.proj.p, X.xo1.p, X.n);
-            copyD(X.u.p, X.yo1.p, X.m);
-            pBest = avg;
-            improved = true;
+        if (!anyCheck) break;
+        std::vector<Metrics> cur, avg;
+        score(cur, avg);
+        std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
+        for (int k = 0; k < B; ++k) {
+            if (!fl[F_MODE][k]) continue;
+            if (cur[k].kkt < I[k].pBest.kkt) {
+                fl[F_SEL][k] = 1;
+                I[k].pBest = cur[k];
+                I[k].pImproved = true;
+            }
+            if (fl[F_MODE][k] == 2 && avg[k].kkt < I[k].pBest.kkt) {
+                fl[F_SEL][k] = 2;
+                I[k].pBest = avg[k];
+                I[k].pImproved = true;
+            }
+            if (optimal(I[k].pBest)) active[k] = 0;
         }
-        if (optimal(pBest)) break;
+        selectInto(X.pbx.p, X.pby.p, X.xo0.p, X.yo0.p, X.xo1.p, X.yo1.p);
+    }
+    std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
+    for (int k = 0; k < B; ++k) {
+        Inst& in = I[k];
+        if (in.polishing && in.pImproved && in.pBest.kkt < in.best.kkt) {
+            fl[F_SEL][k] = 1;
+            in.best = in.pBest;
+        }
+    }
+    selectInto(X.bestx.p, X.besty.p, X.pbx.p, X.pby.p, X.pbx.p, X.pby.p);
+    for (int k = 0; k < B; ++k) {
+        Inst& in = I[k];
+        if (!in.polishing) continue;
+        if (optimal(in.best)) {
+            in.status = PDLPX_OPTIMAL;
+            in.msg = "Tolerances satisfied after refinement polishing";
+            in.mainIts += in.pIts;
+            in.mainTrials += in.pTrials;
+        }
+        in.finishTime = since();
     }
 }
 
-void Run::go(const double* warmX, const double* warmY, pdlpx_summary* sum, double* xOut, double* yOut,
-             double* rayOut) {
-    std::memset(sum, 0, sizeof(*sum));
-    sum->final_primal_weight = 1.0;
-    const int n = X.n, m = X.m;
-    hipEvent_t e0 = X.ev0, e1 = X.ev1;
-    HIPCHK(hipEventRecord(e0, s));
+void Run::collect(pdlpx_summary* const* sums, double* const* xOut, double* const* yOut, double* const* rayOut) {
+    for (int k = 0; k < B; ++k) {
+        const int u = X.dev[k];
+        const InstInfo& in = X.info[u];
+        if (xOut && xOut[u] && in.n)
+            HIPCHK(hipMemcpyAsync(xOut[u], X.bestx.p + X.colOff[k], 8 * size_t(in.n), hipMemcpyDeviceToHost, s));
+        if (yOut && yOut[u] && in.m)
+            HIPCHK(hipMemcpyAsync(yOut[u], X.besty.p + X.rowOff[k], 8 * size_t(in.m), hipMemcpyDeviceToHost, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    for (int k = 0; k < B; ++k) {
+        const int u = X.dev[k];
+        const Inst& in = I[k];
+        pdlpx_summary* sm = sums[u];
+        std::memset(sm, 0, sizeof(*sm));
+        sm->status = in.status;
+        setText(sm, in.msg);
+        sm->primal_objective = in.best.pobj;
+        sm->dual_objective = in.best.dobj;
+        sm->primal_residual = in.best.pres;
+        sm->dual_residual = in.best.dres;
+        sm->relative_gap = in.best.gap;
+        // diagnostics of the main loop (pdlp_solver.cpp:267-272); polishing
+        // iterations count only when polishing reached the tolerances
+        sm->iterations = in.diag ? in.mainIts : 0;
+        sm->step_trials = in.diag ? in.mainTrials : 0;
+        sm->restart_count = in.restarts;
+        sm->final_step_size = in.diag ? in.mainEta : 0.0;
+        sm->static_step_bound = in.diag ? in.mainSafe : 0.0;
+        sm->final_primal_weight = in.diag ? in.mainOmega : 1.0;
+        sm->has_dual_ray = in.rayKind == PDLPX_INFEASIBLE ? 1 : 0;
+        sm->has_primal_ray = in.rayKind == PDLPX_UNBOUNDED ? 1 : 0;
+        sm->fp32_iterations = in.fp32Its;
+        sm->device_loop_seconds = loopSeconds;
+        sm->setup_seconds = setupSeconds;
+        sm->termination_checks = in.checks;
+        sm->solve_time_seconds = in.finishTime > 0.0 ? in.finishTime : since();
+        if (rayOut && rayOut[u] && in.rayKind && !rays[k].empty())
+            std::memcpy(rayOut[u], rays[k].data(), rays[k].size() * sizeof(double));
+    }
+}
+
+void Run::go(const double* const* warmX, const double* const* warmY, pdlpx_summary* const* sums,
+             double* const* xOut, double* const* yOut, double* const* rayOut) {
+    HIPCHK(hipEventRecord(X.ev0, s));
     setup();
-    prec = o.precision == PDLPX_FP32 ? PDLPX_FP32 : PDLPX_FP64;
-
-    // start point (working coordinates, FP64), original copy for scoring
-    const double* dwx = nullptr;
-    const double* dwy = nullptr;
-    if (warmX && n) {
-        HIPCHK(hipMemcpyAsync(X.warmx.p, warmX, sizeof(double) * n, hipMemcpyHostToDevice, s));
-        dwx = X.warmx.p;
-    }
-    if (warmY && m) {
-        HIPCHK(hipMemcpyAsync(X.warmy.p, warmY, sizeof(double) * m, hipMemcpyHostToDevice, s));
-        dwy = X.warmy.p;
-    }
-    IterBuf<double>& I = X.it64;
-    k_start<<<gridFor(n + m), kThreads, 0, s>>>(n, m, X.w64.lo.p, X.w64.hi.p, X.w64.rlo.p, X.w64.rhi.p, X.cs.p,
-                                                X.rs.p, dwx, dwy, I.x[0].p, I.y[0].p);
-    HIPCHK(hipGetLastError());
-    std::memset(&hs, 0, sizeof(hs));
-    resetSteps(maxSafe);
-    setWeight(o.initial_primal_weight);
-    hs.adaptive = o.use_adaptive_linesearch ? 1 : 0;
-    hs.averaging = o.use_averaging ? 1 : 0;
-    hs.max_trials = std::max(o.maximum_step_trials, 1);
-
-    // score the start (pdlp_solver.cpp:239-264)
-    toOriginal<double>(I.x[0].p, I.y[0].p, X.xo0.p, X.yo0.p);
-    copyD(X.xo1.p, X.xo0.p, n);
-    copyD(X.yo1.p, X.yo0.p, m);
-    Metrics m0, m1;
-    scorePair(m0, m1);
-    copyD(X.bestx.p, X.xo0.p, n);
-    copyD(X.besty.p, X.yo0.p, m);
-    best.m = m0;
-    HIPCHK(hipEventRecord(e1, s));
+    initStart(warmX, warmY);
+    HIPCHK(hipEventRecord(X.ev1, s));
     HIPCHK(hipStreamSynchronize(s));
     float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
     setupSeconds = ms * 1e-3;
-
-    int status = PDLPX_ITERATION_LIMIT;
-    std::string msg = "Iteration limit reached";
-    int restarts = 0;
-    bool diag = false;
-    int64_t extraIts = 0, extraTrials = 0;
-    int rayKind = 0;
-
-    auto fetchRay = [&](int kind) { rayKind = kind; };
-
-    if (optimal(best.m)) {
-        status = PDLPX_OPTIMAL;
-        msg = "Initial point satisfies the requested tolerances";
-    } else {
-        diag = true;
-        HIPCHK(hipEventRecord(e0, s));
-        // FP64 working start -> precision of the first phase
-        if (prec == PDLPX_FP32) {
-            IterBuf<float>& F = X.it32;
-            k_convert<float, double><<<streamGrid(n) * 4, kThreads, 0, s>>>(n, I.x[0].p, F.x[0].p);
-            k_convert<float, double><<<streamGrid(m) * 4, kThreads, 0, s>>>(m, I.y[0].p, F.y[0].p);
-            HIPCHK(hipGetLastError());
-            refreshActivity<float>();
-            copyT(F.anx.p, F.x[0].p, n);
-            copyT(F.any.p, F.y[0].p, m);
-            zeroT(F.avx.p, n);
-            zeroT(F.avy.p, m);
-        } else {
-            refreshActivity<double>();
-            copyT(I.anx.p, I.x[0].p, n);
-            copyT(I.any.p, I.y[0].p, m);
-            zeroT(I.avx.p, n);
-            zeroT(I.avy.p, m);
-        }
-        bool finished = false;
-        if (prec == PDLPX_FP32) {
-            finished = mainLoop<float>(status, msg, restarts);
-            if (finished && (status == PDLPX_INFEASIBLE || status == PDLPX_UNBOUNDED)) fetchRay(status);
-            if (!finished && hs.iteration < o.iteration_limit &&
-                !(status == PDLPX_TIME_LIMIT)) {
-                // hand over: continue in FP64 from the best point, as after a restart
-                const int64_t its = hs.iteration, trials = hs.trials;
-                const double eta = hs.eta, omega = hs.omega;
-                adoptOriginal<double>(X.bestx.p, X.besty.p);
-                hs.iteration = its;
-                hs.trials = trials;
-                hs.eta = eta;
-                hs.eta_trial = eta;
-                hs.omega = omega;
-                prec = PDLPX_FP64;
-                finished = mainLoop<double>(status, msg, restarts);
-            }
-        } else {
-            finished = mainLoop<double>(status, msg, restarts);
-        }
-        if (finished && (status == PDLPX_INFEASIBLE || status == PDLPX_UNBOUNDED)) fetchRay(status);
-        const int64_t mainIts = hs.iteration, mainTrials = hs.trials;
-        const double mainEta = hs.eta, mainOmega = hs.omega, mainSafe = hs.max_safe;
-        HIPCHK(hipEventRecord(e1, s));
-        HIPCHK(hipStreamSynchronize(s));
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        loopSeconds = ms * 1e-3;
-        if (!finished && o.use_feasibility_polishing && n > 0) {
-            const bool expired = o.time_limit_seconds > 0.0 && o.time_limit_seconds - since() <= 0.0;
-            if (!expired) {
-                int64_t pIts = 0, pTrials = 0;
-                Metrics pBest = best.m;
-                bool improved = false;
-                if (prec == PDLPX_FP32) polish<float>(pIts, pTrials, pBest, improved);
-                else polish<double>(pIts, pTrials, pBest, improved);
-                if (pBest.kkt < best.m.kkt && improved) {
-                    copyD(X.bestx.p, X.proj.p, n);
-                    copyD(X.besty.p, X.u.p, m);
-                    best.m = pBest;
-                }
-                if (optimal(best.m)) {
-                    status = PDLPX_OPTIMAL;
-                    msg = "Tolerances satisfied after refinement polishing";
-                    extraIts = pIts;
-                    extraTrials = pTrials;
-                }
-            }
-        }
-        hs.iteration = mainIts + extraIts;
-        hs.trials = mainTrials + extraTrials;
-        hs.eta = mainEta;
-        hs.omega = mainOmega;
-        hs.max_safe = mainSafe;
-    }
-
-    // results (pdlp_solver.cpp:43-68,267-272)
-    if (n && xOut) HIPCHK(hipMemcpyAsync(xOut, X.bestx.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    if (m && yOut) HIPCHK(hipMemcpyAsync(yOut, X.besty.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-    if (rayOut && rayKind == PDLPX_INFEASIBLE && m)
-        HIPCHK(hipMemcpyAsync(rayOut, X.ry.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
-    if (rayOut && rayKind == PDLPX_UNBOUNDED && n)
-        HIPCHK(hipMemcpyAsync(rayOut, X.rx.p, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(X.ev0, s));
+    mainLoop();
+    HIPCHK(hipEventRecord(X.ev1, s));
     HIPCHK(hipStreamSynchronize(s));
-    sum->status = status;
-    setStatusText(sum, msg);
-    sum->primal_objective = best.m.pobj;
-    sum->dual_objective = best.m.dobj;
-    sum->primal_residual = best.m.pres;
-    sum->dual_residual = best.m.dres;
-    sum->relative_gap = best.m.gap;
-    sum->iterations = diag ? hs.iteration : 0;
-    sum->step_trials = diag ? hs.trials : 0;
-    sum->restart_count = restarts;
-    sum->final_step_size = diag ? hs.eta : 0.0;
-    sum->static_step_bound = diag ? hs.max_safe : 0.0;
-    sum->final_primal_weight = diag ? hs.omega : 1.0;
-    sum->has_dual_ray = rayKind == PDLPX_INFEASIBLE ? 1 : 0;
-    sum->has_primal_ray = rayKind == PDLPX_UNBOUNDED ? 1 : 0;
-    sum->fp32_iterations = fp32Its;
-    sum->device_loop_seconds = loopSeconds;
-    sum->setup_seconds = setupSeconds;
-    sum->termination_checks = checks;
-    sum->solve_time_seconds = since();
+    HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
+    loopSeconds = ms * 1e-3;
+    polishStage();
+    collect(sums, xOut, yOut, rayOut);
 }
 
-void allocWork(pdlpx_ctx& X) {
-    const int n = X.n, m = X.m;
+// ---------------------------------------------------------------------------
+// Loading
+// ---------------------------------------------------------------------------
+void uploadOrient(OrientBuf& ob, const std::vector<int32_t>& ptr, const std::vector<int32_t>& instStart,
+                  const std::vector<int32_t>& idx, const std::vector<double>& val, hipStream_t s) {
+    std::vector<int32_t> blk, binst, ibeg;
+    partitionRows(ptr, instStart, blk, binst, ibeg);
+    ob.nrows = static_cast<int32_t>(ptr.size()) - 1;
+    ob.nblk = static_cast<int32_t>(binst.size());
+    ob.ptr.upload(ptr.data(), ptr.size(), s);
+    ob.blk.upload(blk.data(), blk.size(), s);
+    ob.binst.upload(binst.data(), binst.size(), s);
+    ob.ibeg.upload(ibeg.data(), ibeg.size(), s);
+    ob.idx.upload(idx.data(), idx.size(), s);
+    ob.val.upload(val.data(), val.size(), s);
+}
+
+void allocWork(pdlpx_ctx& X, bool fp32) {
+    const int n = X.n, m = X.m, B = X.B;
     const size_t nz = static_cast<size_t>(X.nnz);
-    for (auto* b : {&X.rs, &X.rmax, &X.yo0, &X.yo1, &X.besty, &X.ry, &X.u, &X.warmy}) b->alloc(m);
-    for (auto* b : {&X.cs, &X.cmax, &X.xo0, &X.xo1, &X.bestx, &X.rx, &X.proj, &X.warmx}) b->alloc(n);
-    // power iteration reuses xo0 (n), yo0 (m), xo1 (n)
+    for (auto* b : {&X.rs, &X.rmax, &X.yo0, &X.yo1, &X.besty, &X.pby, &X.ry, &X.u, &X.warmy}) b->alloc(m);
+    for (auto* b : {&X.cs, &X.cmax, &X.xo0, &X.xo1, &X.bestx, &X.pbx, &X.rx, &X.proj, &X.warmx}) b->alloc(n);
     Working<double>& w = X.w64;
     w.vr.alloc(nz);
     w.vc.alloc(nz);
-    for (auto* b : {&w.c, &w.lo, &w.hi, &w.tau, &w.tau_inv}) b->alloc(n);
-    for (auto* b : {&w.rlo, &w.rhi, &w.sig, &w.sig_inv}) b->alloc(m);
+    for (auto* b : {&w.c, &w.lo, &w.hi, &w.tau}) b->alloc(n);
+    for (auto* b : {&w.rlo, &w.rhi, &w.sig}) b->alloc(m);
     X.it64.alloc(n, m);
     X.part_p.alloc(X.AT.nblk);
     X.part_d.alloc(X.A.nblk);
     X.part_i.alloc(X.A.nblk);
     X.part_rows.alloc(6 * size_t(X.A.nblk));
     X.part_cols.alloc(6 * size_t(X.AT.nblk));
-    X.part_s.alloc(2 * kStreamBlocks);
-    X.part_s2.alloc(2 * kStreamBlocks);
-    X.part_s3.alloc(2 * kStreamBlocks);
-    X.red.alloc(32);
-    X.st.alloc(1);
+    const size_t ns = size_t(X.ncs + X.nrs);
+    X.part_seg.alloc(2 * ns);
+    X.part_seg2.alloc(2 * ns);
+    X.part_seg3.alloc(2 * ns);
+    X.red.alloc(24 * size_t(B));
+    X.flags.alloc(F_COUNT * size_t(B));
+    X.st.alloc(B);
+    if (fp32) {
+        Working<float>& h = X.w32;
+        h.vr.alloc(nz);
+        h.vc.alloc(nz);
+        for (auto* b : {&h.c, &h.lo, &h.hi, &h.tau}) b->alloc(n);
+        for (auto* b : {&h.rlo, &h.rhi, &h.sig}) b->alloc(m);
+        X.it32.alloc(n, m);
+        X.have32 = true;
+    }
 }
 
-void allocFp32(pdlpx_ctx& X) {
-    const int n = X.n, m = X.m;
-    const size_t nz = static_cast<size_t>(X.nnz);
-    Working<float>& h = X.w32;
-    h.vr.alloc(nz);
-    h.vc.alloc(nz);
-    for (auto* b : {&h.c, &h.lo, &h.hi, &h.tau, &h.tau_inv}) b->alloc(n);
-    for (auto* b : {&h.rlo, &h.rhi, &h.sig, &h.sig_inv}) b->alloc(m);
-    X.it32.alloc(n, m);
+void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t precision) {
+    X.dropGraphs();
+    X.loaded = true;
+    X.info.assign(count, InstInfo{});
+    X.dev.clear();
+    for (int u = 0; u < count; ++u) {
+        InstInfo& in = X.info[u];
+        in.invalid = validateProblem(probs[u]);
+        in.valid = in.invalid.empty();
+        if (!in.valid) continue;
+        const pdlpx_problem& p = probs[u];
+        in.m = p.rows;
+        in.n = p.cols;
+        in.nnz = p.nnz;
+        in.offset = p.objective_offset;
+        double sq = 0.0;   // termination.cpp:15-29 in the reference's order
+        for (int i = 0; i < p.rows; ++i) {
+            if (std::isfinite(p.row_lower[i])) sq += p.row_lower[i] * p.row_lower[i];
+            if (std::isfinite(p.row_upper[i]) && p.row_upper[i] != p.row_lower[i]) sq += p.row_upper[i] * p.row_upper[i];
+        }
+        in.pnorm = 1.0 + std::sqrt(sq);
+        double cq = 0.0;
+        for (int j = 0; j < p.cols; ++j) cq += p.objective[j] * p.objective[j];
+        in.dnorm = 1.0 + std::sqrt(cq);
+        X.dev.push_back(u);
+    }
+    X.B = static_cast<int>(X.dev.size());
+    const int B = X.B;
+    X.rowOff.assign(B + 1, 0);
+    X.colOff.assign(B + 1, 0);
+    int64_t total = 0;
+    for (int k = 0; k < B; ++k) {
+        const InstInfo& in = X.info[X.dev[k]];
+        if (int64_t(X.rowOff[k]) + in.m > INT32_MAX || int64_t(X.colOff[k]) + in.n > INT32_MAX)
+            throw Failure{PDLPX_E_INVALID, "batch too large for 32-bit device indices"};
+        X.rowOff[k + 1] = X.rowOff[k] + in.m;
+        X.colOff[k + 1] = X.colOff[k] + in.n;
+        total += in.nnz;
+    }
+    if (total >= (int64_t(1) << 31)) throw Failure{PDLPX_E_INVALID, "Matrix has too many nonzeros for 32-bit device offsets"};
+    X.m = X.rowOff[B];
+    X.n = X.colOff[B];
+    X.nnz = total;
+    if (B == 0) return;
+    // concatenated CSR / CSC with global indices
+    std::vector<int32_t> rp(X.m + 1), cp(X.n + 1), ci(total), ri(total);
+    std::vector<double> rv(total), cv(total), c(X.n), lo(X.n), hi(X.n), rlo(X.m), rhi(X.m);
+    int64_t zr = 0, zc = 0;
+    for (int k = 0; k < B; ++k) {
+        const pdlpx_problem& p = probs[X.dev[k]];
+        const int R = X.rowOff[k], Cc = X.colOff[k];
+        for (int i = 0; i < p.rows; ++i) rp[R + i] = static_cast<int32_t>(zr + p.csr_row_start[i]);
+        for (int j = 0; j < p.cols; ++j) cp[Cc + j] = static_cast<int32_t>(zc + p.csc_col_start[j]);
+        for (int64_t q = 0; q < p.nnz; ++q) {
+            ci[zr + q] = Cc + p.csr_col[q];
+            rv[zr + q] = p.csr_val[q];
+            ri[zc + q] = R + p.csc_row[q];
+            cv[zc + q] = p.csc_val[q];
+        }
+        zr += p.nnz;
+        zc += p.nnz;
+        std::copy(p.objective, p.objective + p.cols, c.begin() + Cc);
+        std::copy(p.var_lower, p.var_lower + p.cols, lo.begin() + Cc);
+        std::copy(p.var_upper, p.var_upper + p.cols, hi.begin() + Cc);
+        std::copy(p.row_lower, p.row_lower + p.rows, rlo.begin() + R);
+        std::copy(p.row_upper, p.row_upper + p.rows, rhi.begin() + R);
+    }
+    rp[X.m] = static_cast<int32_t>(zr);
+    cp[X.n] = static_cast<int32_t>(zc);
+    hipStream_t s = X.stream;
+    uploadOrient(X.A, rp, X.rowOff, ci, rv, s);
+    uploadOrient(X.AT, cp, X.colOff, ri, cv, s);
+    X.c.upload(c.data(), c.size(), s);
+    X.lo.upload(lo.data(), lo.size(), s);
+    X.hi.upload(hi.data(), hi.size(), s);
+    X.rlo.upload(rlo.data(), rlo.size(), s);
+    X.rhi.upload(rhi.data(), rhi.size(), s);
+    std::vector<Seg> segs;
+    std::vector<int32_t> cbeg, rbeg;
+    makeSegments(X.colOff, segs, cbeg);
+    X.ncs = static_cast<int>(segs.size());
+    makeSegments(X.rowOff, segs, rbeg);
+    X.nrs = static_cast<int>(segs.size()) - X.ncs;
+    std::vector<int32_t> rall(rbeg);
+    for (auto& v : rall) v += X.ncs;
+    X.segs.upload(segs.data(), segs.size(), s);
+    X.colSegBeg.upload(cbeg.data(), cbeg.size(), s);
+    X.rowSegBeg.upload(rbeg.data(), rbeg.size(), s);
+    X.rowSegBegAll.upload(rall.data(), rall.size(), s);
+    allocWork(X, precision == PDLPX_FP32);
+    HIPCHK(hipStreamSynchronize(s));   // host staging vectors die here
 }
 
 template <typename F>
@@ -1085,6 +1428,35 @@ int guarded(pdlpx_ctx* X, F&& body) {
         X->error = e.what();
         return PDLPX_E_INVALID;
     }
+}
+
+void rejectAll(pdlpx_summary* sm, const std::string& why, double seconds) {
+    std::memset(sm, 0, sizeof(*sm));
+    sm->status = PDLPX_INVALID_PROBLEM;
+    setText(sm, why);
+    sm->primal_objective = std::numeric_limits<double>::quiet_NaN();
+    sm->dual_objective = -kInf;
+    sm->primal_residual = sm->dual_residual = sm->relative_gap = kInf;
+    sm->final_primal_weight = 1.0;
+    sm->solve_time_seconds = seconds;
+}
+
+// Solves every loaded instance; per-caller-index arrays (any may be null).
+void solveLoaded(pdlpx_ctx& X, const pdlpx_options& o, const double* const* wx, const double* const* wy,
+                 pdlpx_summary* const* sums, double* const* xOut, double* const* yOut, double* const* rayOut) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto secs = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    const int users = static_cast<int>(X.info.size());
+    if (!optionsValid(o)) {
+        for (int u = 0; u < users; ++u) rejectAll(sums[u], "Invalid PDLP options", secs());
+        return;
+    }
+    for (int u = 0; u < users; ++u)
+        if (!X.info[u].valid) rejectAll(sums[u], X.info[u].invalid, secs());
+    if (X.B == 0) return;
+    if (o.precision == PDLPX_FP32 && !X.have32) allocWork(X, true);
+    Run run(X, o);
+    run.go(wx, wy, sums, xOut, yOut, rayOut);
 }
 
 }  // namespace
@@ -1135,16 +1507,18 @@ int pdlpx_create(pdlpx_ctx** out, int device) {
     auto* X = new (std::nothrow) pdlpx_ctx();
     if (!X) return PDLPX_E_NOMEM;
     X->device = device;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+        delete X;
+        return PDLPX_E_INVALID;
+    }
     const int rc = guarded(X, [&] {
-        int count = 0;
-        HIPCHK(hipGetDeviceCount(&count));
-        if (device < 0 || device >= count) throw Failure{PDLPX_E_INVALID, "no such HIP device"};
         HIPCHK(hipStreamCreateWithFlags(&X->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&X->ev0));
         HIPCHK(hipEventCreate(&X->ev1));
     });
     if (rc != PDLPX_OK) {
-        delete X;
+        pdlpx_destroy(X);
         return rc;
     }
     *out = X;
@@ -1166,81 +1540,69 @@ const char* pdlpx_last_error(const pdlpx_ctx* X) { return X ? X->error.c_str() :
 
 int pdlpx_load(pdlpx_ctx* X, const pdlpx_problem* p, int32_t precision) {
     if (!p) return PDLPX_E_INVALID;
-    return guarded(X, [&] {
-        X->dropGraphs();
-        X->loaded = true;
-        X->invalid = validateProblem(*p);
-        X->valid = X->invalid.empty();
-        if (!X->valid) return;
-        hipStream_t s = X->stream;
-        X->m = p->rows;
-        X->n = p->cols;
-        X->nnz = p->nnz;
-        X->offset = p->objective_offset;
-        uploadOrient(X->A, p->rows, p->csr_row_start, p->csr_col, p->csr_val, p->nnz, s);
-        uploadOrient(X->AT, p->cols, p->csc_col_start, p->csc_row, p->csc_val, p->nnz, s);
-        X->c.upload(p->objective, p->cols, s);
-        X->lo.upload(p->var_lower, p->cols, s);
-        X->hi.upload(p->var_upper, p->cols, s);
-        X->rlo.upload(p->row_lower, p->rows, s);
-        X->rhi.upload(p->row_upper, p->rows, s);
-        // termination normalisers (termination.cpp:15-37,60-61), host order
-        double sq = 0.0;
-        for (int i = 0; i < p->rows; ++i) {
-            if (std::isfinite(p->row_lower[i])) sq += p->row_lower[i] * p->row_lower[i];
-            if (std::isfinite(p->row_upper[i]) && p->row_upper[i] != p->row_lower[i])
-                sq += p->row_upper[i] * p->row_upper[i];
-        }
-        X->pnorm = 1.0 + std::sqrt(sq);
-        double cq = 0.0;
-        for (int j = 0; j < p->cols; ++j) cq += p->objective[j] * p->objective[j];
-        X->dnorm = 1.0 + std::sqrt(cq);
-        allocWork(*X);
-        if (precision == PDLPX_FP32) allocFp32(*X);
-        HIPCHK(hipStreamSynchronize(s));
-    });
+    return guarded(X, [&] { loadBatch(*X, 1, p, precision); });
+}
+
+int pdlpx_load_batch(pdlpx_ctx* X, int32_t count, const pdlpx_problem* problems, int32_t precision) {
+    if (count < 0 || (count > 0 && !problems)) return PDLPX_E_INVALID;
+    return guarded(X, [&] { loadBatch(*X, count, problems, precision); });
 }
 
 int pdlpx_solve(pdlpx_ctx* X, const pdlpx_options* options, const double* warm_x, const double* warm_y,
                 pdlpx_summary* summary, double* x_out, double* y_out, double* ray_out) {
     if (!options || !summary) return PDLPX_E_INVALID;
-    const auto t0 = std::chrono::steady_clock::now();
     return guarded(X, [&] {
-        std::memset(summary, 0, sizeof(*summary));
-        summary->final_primal_weight = 1.0;
         if (!X->loaded) throw Failure{PDLPX_E_STATE, "pdlpx_solve called before pdlpx_load"};
-        auto reject = [&](const std::string& why) {
-            summary->status = PDLPX_INVALID_PROBLEM;
-            setStatusText(summary, why);
-            summary->primal_objective = std::numeric_limits<double>::quiet_NaN();
-            summary->dual_objective = -kInf;
-            summary->primal_residual = summary->dual_residual = summary->relative_gap = kInf;
-            summary->solve_time_seconds =
-                std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        };
-        if (!X->valid) return reject(X->invalid);
-        if (!optionsValid(*options)) return reject("Invalid PDLP options");
-        if (options->precision == PDLPX_FP32 && !X->w32.vr.p) allocFp32(*X);
-        Run run(*X, *options);
-        run.go(warm_x, warm_y, summary, x_out, y_out, ray_out);
+        if (X->info.size() != 1) throw Failure{PDLPX_E_STATE, "pdlpx_solve needs exactly one loaded LP"};
+        pdlpx_summary* sums[1] = {summary};
+        double* xs[1] = {x_out};
+        double* ys[1] = {y_out};
+        double* rays[1] = {ray_out};
+        const double* wx[1] = {warm_x};
+        const double* wy[1] = {warm_y};
+        solveLoaded(*X, *options, wx, wy, sums, xs, ys, rays);
+    });
+}
+
+int pdlpx_solve_loaded(pdlpx_ctx* X, const pdlpx_options* options, const double* const* warm_x,
+                       const double* const* warm_y, pdlpx_summary* summaries, double* const* x_out,
+                       double* const* y_out) {
+    if (!options || !summaries) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded) throw Failure{PDLPX_E_STATE, "pdlpx_solve_loaded called before a load"};
+        std::vector<pdlpx_summary*> sums(X->info.size());
+        for (size_t u = 0; u < sums.size(); ++u) sums[u] = summaries + u;
+        solveLoaded(*X, *options, warm_x, warm_y, sums.data(), x_out, y_out, nullptr);
+    });
+}
+
+int pdlpx_solve_batch(pdlpx_ctx* X, int32_t count, const pdlpx_problem* problems, const pdlpx_options* options,
+                      pdlpx_summary* summaries, double* const* x_out, double* const* y_out) {
+    if (count < 0 || (count > 0 && (!problems || !summaries || !options))) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        loadBatch(*X, count, problems, options->precision);
+        std::vector<pdlpx_summary*> sums(count);
+        for (int u = 0; u < count; ++u) sums[u] = summaries + u;
+        solveLoaded(*X, *options, nullptr, nullptr, sums.data(), x_out, y_out, nullptr);
     });
 }
 
 int pdlpx_spmv(pdlpx_ctx* X, int32_t transpose, const double* x, double* y) {
     if (!x || !y) return PDLPX_E_INVALID;
     return guarded(X, [&] {
-        if (!X->loaded || !X->valid) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
         hipStream_t s = X->stream;
         const int nin = transpose ? X->m : X->n;
         const int nout = transpose ? X->n : X->m;
-        double* din = transpose ? X->yo0.p : X->xo0.p;
-        double* dout = transpose ? X->xo1.p : X->yo1.p;
-        if (nin) HIPCHK(hipMemcpyAsync(din, x, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+        DBuf<double> din, dout;
+        din.alloc(nin);
+        dout.alloc(nout);
+        if (nin) HIPCHK(hipMemcpyAsync(din.p, x, sizeof(double) * nin, hipMemcpyHostToDevice, s));
         const OrientBuf& M = transpose ? X->AT : X->A;
         if (nout) {
-            k_spmv<double><<<M.nblk, kThreads, 0, s>>>(M.view<double>(M.val.p), din, dout);
+            k_spmv<double><<<M.nblk, kThreads, 0, s>>>(M.view<double>(M.val.p), din.p, dout.p);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(y, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(y, dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
         }
         HIPCHK(hipStreamSynchronize(s));
     });
@@ -1249,10 +1611,9 @@ int pdlpx_spmv(pdlpx_ctx* X, int32_t transpose, const double* x, double* y) {
 int pdlpx_spmv_device(pdlpx_ctx* X, int32_t transpose, const void* x, void* y) {
     if (!x || !y) return PDLPX_E_INVALID;
     return guarded(X, [&] {
-        if (!X->loaded || !X->valid) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
         const OrientBuf& M = transpose ? X->AT : X->A;
-        k_spmv<double><<<M.nblk, kThreads, 0, X->stream>>>(M.view<double>(M.val.p),
-                                                            static_cast<const double*>(x),
+        k_spmv<double><<<M.nblk, kThreads, 0, X->stream>>>(M.view<double>(M.val.p), static_cast<const double*>(x),
                                                             static_cast<double*>(y));
         HIPCHK(hipGetLastError());
     });
@@ -1261,7 +1622,7 @@ int pdlpx_spmv_device(pdlpx_ctx* X, int32_t transpose, const void* x, void* y) {
 int pdlpx_scaling(pdlpx_ctx* X, int32_t ruiz_iterations, int32_t use_pc, double* row_scale, double* col_scale,
                   double* primal_scale, double* dual_scale) {
     return guarded(X, [&] {
-        if (!X->loaded || !X->valid) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
         pdlpx_options o;
         pdlpx_default_options(&o);
         o.ruiz_iterations = ruiz_iterations;
@@ -1271,10 +1632,11 @@ int pdlpx_scaling(pdlpx_ctx* X, int32_t ruiz_iterations, int32_t use_pc, double*
         run.setupOnly();
         hipStream_t s = X->stream;
         const int m = X->m, n = X->n;
-        if (row_scale && m) HIPCHK(hipMemcpyAsync(row_scale, X->rs.p, 8 * m, hipMemcpyDeviceToHost, s));
-        if (col_scale && n) HIPCHK(hipMemcpyAsync(col_scale, X->cs.p, 8 * n, hipMemcpyDeviceToHost, s));
-        if (primal_scale && n) HIPCHK(hipMemcpyAsync(primal_scale, X->w64.tau.p, 8 * n, hipMemcpyDeviceToHost, s));
-        if (dual_scale && m) HIPCHK(hipMemcpyAsync(dual_scale, X->w64.sig.p, 8 * m, hipMemcpyDeviceToHost, s));
+        if (row_scale && m) HIPCHK(hipMemcpyAsync(row_scale, X->rs.p, 8 * size_t(m), hipMemcpyDeviceToHost, s));
+        if (col_scale && n) HIPCHK(hipMemcpyAsync(col_scale, X->cs.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+        if (primal_scale && n)
+            HIPCHK(hipMemcpyAsync(primal_scale, X->w64.tau.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
+        if (dual_scale && m) HIPCHK(hipMemcpyAsync(dual_scale, X->w64.sig.p, 8 * size_t(m), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
     });
 }
@@ -1282,16 +1644,21 @@ int pdlpx_scaling(pdlpx_ctx* X, int32_t ruiz_iterations, int32_t use_pc, double*
 int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_primal, double* ms_dual) {
     if (!ms_primal || !ms_dual || reps <= 0) return PDLPX_E_INVALID;
     return guarded(X, [&] {
-        if (!X->loaded || !X->valid) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
-        if (precision == PDLPX_FP32 && !X->w32.vr.p) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        if (precision == PDLPX_FP32 && !X->have32) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
         hipStream_t s = X->stream;
-        DevState saved{}, probe{};
-        HIPCHK(hipMemcpyAsync(&saved, X->st.p, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        const int B = X->B;
+        std::vector<DevState> saved(B), probe;
+        HIPCHK(hipMemcpyAsync(saved.data(), X->st.p, sizeof(DevState) * B, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         probe = saved;
-        probe.done = 0;
-        HIPCHK(hipMemcpyAsync(X->st.p, &probe, sizeof(DevState), hipMemcpyHostToDevice, s));
+        for (auto& h : probe) {
+            h.done = 0;
+            h.phase = precision == PDLPX_FP32 ? 1 : 0;
+        }
+        HIPCHK(hipMemcpyAsync(X->st.p, probe.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
         auto timeIt = [&](auto&& launch) {
+            launch();   // warm
             HIPCHK(hipEventRecord(X->ev0, s));
             for (int r = 0; r < reps; ++r) launch();
             HIPCHK(hipEventRecord(X->ev1, s));
@@ -1300,51 +1667,34 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
             HIPCHK(hipEventElapsedTime(&ms, X->ev0, X->ev1));
             return static_cast<double>(ms) / reps;
         };
-        if (precision == PDLPX_FP32) {
-            Iterates<float> it = X->it32.view();
-            const Working<float>& w = X->w32;
-            ColData<float> cd{w.c.p, w.lo.p, w.hi.p, w.tau.p, w.tau_inv.p};
-            RowData<float> rd{w.rlo.p, w.rhi.p, w.sig.p, w.sig_inv.p};
+        auto both = [&](auto tag) {
+            using T = decltype(tag);
+            const Working<T>* wp;
+            const IterBuf<T>* ip;
+            if constexpr (sizeof(T) == 4) {
+                wp = &X->w32;
+                ip = &X->it32;
+            } else {
+                wp = &X->w64;
+                ip = &X->it64;
+            }
+            const Working<T>& w = *wp;
+            const IterBuf<T>& it = *ip;
             *ms_primal = timeIt([&] {
-                k_primal<float><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<float>(w.vc.p), cd, it, X->st.p, X->part_p.p);
+                k_primal<T><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<T>(w.vc.p), w.cols(), it.view(), X->st.p,
+                                                            X->part_p.p);
             });
             *ms_dual = timeIt([&] {
-                k_dual<float><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<float>(w.vr.p), rd, it, X->st.p, X->part_d.p,
-                                                             X->part_i.p);
+                k_dual<T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<T>(w.vr.p), w.rows(), it.view(), X->st.p,
+                                                         X->part_d.p, X->part_i.p);
             });
-        } else {
-            Iterates<double> it = X->it64.view();
-            const Working<double>& w = X->w64;
-            ColData<double> cd{w.c.p, w.lo.p, w.hi.p, w.tau.p, w.tau_inv.p};
-            RowData<double> rd{w.rlo.p, w.rhi.p, w.sig.p, w.sig_inv.p};
-            *ms_primal = timeIt([&] {
-                k_primal<double><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<double>(w.vc.p), cd, it, X->st.p,
-                                                                 X->part_p.p);
-            });
-            *ms_dual = timeIt([&] {
-                k_dual<double><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<double>(w.vr.p), rd, it, X->st.p,
-                                                              X->part_d.p, X->part_i.p);
-            });
-        }
+        };
+        if (precision == PDLPX_FP32) both(float{});
+        else both(double{});
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(X->st.p, &saved, sizeof(DevState), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(X->st.p, saved.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
     });
-}
-
-int pdlpx_solve_batch(pdlpx_ctx* X, int32_t count, const pdlpx_problem* problems, const pdlpx_options* options,
-                      pdlpx_summary* summaries, double* const* x_out, double* const* y_out) {
-    if (count < 0 || (count > 0 && (!problems || !summaries || !options))) return PDLPX_E_INVALID;
-    // Sequential reference implementation of the batch entry; the batched
-    // persistent kernel replaces this.
-    for (int k = 0; k < count; ++k) {
-        int rc = pdlpx_load(X, &problems[k], options->precision);
-        if (rc != PDLPX_OK) return rc;
-        rc = pdlpx_solve(X, options, nullptr, nullptr, &summaries[k], x_out ? x_out[k] : nullptr,
-                         y_out ? y_out[k] : nullptr, nullptr);
-        if (rc != PDLPX_OK) return rc;
-    }
-    return PDLPX_OK;
 }
 
 }  // extern "C"

@@ -1,9 +1,17 @@
-// PDLP device kernels for gfx950.  Every kernel here is launched by engine.hip.
+// PDLP device kernels for gfx950, batch-aware.  Every kernel here is launched
+// by engine.hip.
+//
+// A launch covers a batch of B independent LPs stored back to back (instance
+// k owns rows [R_k, R_k+1), columns [C_k, C_k+1) and the matching nonzeros,
+// with global indices).  Row blocks of the sparse passes and the "segments" of
+// the streaming passes never straddle two instances, so every block knows its
+// instance and every per-instance reduction is a contiguous range of per-block
+// partials, summed in a fixed order.  A single LP is the batch B = 1.
 //
 // Naming: "working" coordinates are the Ruiz-scaled problem the iteration runs
 // on; "original" coordinates are the caller's.  T is the iteration precision
-// (double, or float in the mixed-precision phase); every reduction is
-// accumulated in double.
+// (double, or float in the mixed-precision phase); reductions accumulate in
+// double.
 #pragma once
 
 #include "device_common.hpp"
@@ -13,9 +21,11 @@
 
 namespace pdlpx_dev {
 
-// Device-resident controller state.  The trial / decide / average kernels of
-// one PDHG iteration read and write it, so the adaptive linesearch is decided
-// on the device without a host round trip (step_controller.cpp:30-68,
+constexpr int kSegLen = 2048;   // elements per streaming segment
+
+// Per-instance controller state.  The trial / decide / average kernels of one
+// PDHG iteration read and write it, so the adaptive linesearch is decided on
+// the device without a host round trip (step_controller.cpp:30-68,
 // pdlp_solver.cpp:292-348).
 struct DevState {
     double eta;          // the step controller's eta
@@ -30,13 +40,35 @@ struct DevState {
     int64_t target;      // trials stop once iteration reaches this
     int64_t trials;
     int32_t parity;      // which of the two buffers holds the current iterate
-    int32_t done;
+    int32_t done;        // no more trials for this instance in this batch
     int32_t fail;        // 1 = non-finite trial, 2 = linesearch exhausted
     int32_t attempts;    // rejected trials in the current iteration
     int32_t avg_pending;
     int32_t adaptive;
     int32_t averaging;
     int32_t max_trials;
+    int32_t phase;       // 0 = FP64 iterates, 1 = FP32 iterates
+    int32_t pad;
+};
+
+// Streaming segment: elements [begin, end) of one instance's column range
+// (the first ncs segments) or row range (the rest).
+struct Seg {
+    int32_t inst;
+    int32_t begin;
+    int32_t end;
+    int32_t pad;
+};
+
+template <typename T>
+struct PhaseOf;
+template <>
+struct PhaseOf<double> {
+    static constexpr int value = 0;
+};
+template <>
+struct PhaseOf<float> {
+    static constexpr int value = 1;
 };
 
 template <typename T>
@@ -45,7 +77,6 @@ struct ColData {   // per-column data of the working problem
     const T* lo;
     const T* hi;
     const T* tau;
-    const T* tau_inv;
 };
 
 template <typename T>
@@ -53,7 +84,6 @@ struct RowData {   // per-row data of the working problem
     const T* lo;
     const T* hi;
     const T* sig;
-    const T* sig_inv;
 };
 
 template <typename T>
@@ -63,6 +93,8 @@ struct Iterates {  // double-buffered state; index `parity` is the current one
     T* ax[2];
     T* avg_x;
     T* avg_y;
+    T* anc_x;      // restart anchors
+    T* anc_y;
 };
 
 template <typename T>
@@ -77,14 +109,15 @@ __device__ inline float floor_step<float>() { return 1e-37f; }
 // accumulates sum dx_j^2 / T_j.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_primal(Orient<T> at, ColData<T> cd, Iterates<T> it,
-                                                     const DevState* __restrict__ st,
+                                                     const DevState* __restrict__ states,
                                                      double* __restrict__ part) {
-    if (st->done) return;
+    const DevState& st = states[at.binst[blockIdx.x]];
+    if (st.done || st.phase != PhaseOf<T>::value) return;
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     __shared__ double sh[kWaves];
-    const int p = st->parity;
-    const T mul = static_cast<T>(st->eta_trial / st->omega);
+    const int p = st.parity;
+    const T mul = static_cast<T>(st.eta_trial / st.omega);
     const T* __restrict__ x = it.x[p];
     const T* __restrict__ y = it.y[p];
     T* __restrict__ xt = it.x[p ^ 1];
@@ -92,10 +125,11 @@ __global__ __launch_bounds__(kThreads) void k_primal(Orient<T> at, ColData<T> cd
     auto epi = [&](int j, T s) {
         const T g = cd.c[j] + s;
         const T was = x[j];
-        const T now = clip(was - mul * cd.tau[j] * g, cd.lo[j], cd.hi[j]);
+        const T tau = cd.tau[j];
+        const T now = clip(was - mul * tau * g, cd.lo[j], cd.hi[j]);
         xt[j] = now;
         const T d = now - was;
-        acc += static_cast<double>(d * d * cd.tau_inv[j]);
+        acc += static_cast<double>(d * d * (T(1) / tau));
     };
     row_block(at, y, lds, shT, epi);
     double v[1] = {acc};
@@ -109,15 +143,15 @@ __global__ __launch_bounds__(kThreads) void k_primal(Orient<T> at, ColData<T> cd
 // sum dy * (A dx).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_dual(Orient<T> a, RowData<T> rd, Iterates<T> it,
-                                                   const DevState* __restrict__ st,
-                                                   double* __restrict__ part_d,
-                                                   double* __restrict__ part_i) {
-    if (st->done) return;
+                                                   const DevState* __restrict__ states,
+                                                   double* __restrict__ part_d, double* __restrict__ part_i) {
+    const DevState& st = states[a.binst[blockIdx.x]];
+    if (st.done || st.phase != PhaseOf<T>::value) return;
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     __shared__ double sh[2 * kWaves];
-    const int p = st->parity;
-    const T mul = static_cast<T>(st->eta_trial * st->omega);
+    const int p = st.parity;
+    const T mul = static_cast<T>(st.eta_trial * st.omega);
     const T* __restrict__ xt = it.x[p ^ 1];
     const T* __restrict__ y = it.y[p];
     const T* __restrict__ ax = it.ax[p];
@@ -127,14 +161,15 @@ __global__ __launch_bounds__(kThreads) void k_dual(Orient<T> a, RowData<T> rd, I
     auto epi = [&](int i, T act) {
         const T dact = act - ax[i];
         const T ext = act + dact;
-        const T step = smax(mul * rd.sig[i], floor_step<T>());
+        const T sig = rd.sig[i];
+        const T step = smax(mul * sig, floor_step<T>());
         const T was = y[i];
         const T v = was + step * ext;
         const T now = smax(v - step * rd.hi[i], T(0)) + smin(v - step * rd.lo[i], T(0));
         yt[i] = now;
         axt[i] = act;
         const T d = now - was;
-        mov += static_cast<double>(d * d * rd.sig_inv[i]);
+        mov += static_cast<double>(d * d * (T(1) / sig));
         inter += static_cast<double>(d) * static_cast<double>(dact);
     };
     row_block(a, xt, lds, shT, epi);
@@ -172,13 +207,18 @@ __device__ inline double dclamp(double v, double lo, double hi) {   // std::clam
     return v < lo ? lo : (hi < v ? hi : v);
 }
 
-// Linesearch decision and commit (step_controller.cpp:30-68,
-// pdlp_solver.cpp:292-348, iterate_average.cpp:205-247).
-__global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* st, const double* __restrict__ pp,
-                                                            int np, const double* __restrict__ pd,
-                                                            const double* __restrict__ pi, int nd) {
-    __shared__ double sh[3 * (kReduceThreads / 64)];
+// Linesearch decision and commit, one workgroup per instance
+// (step_controller.cpp:30-68, pdlp_solver.cpp:292-348,
+// iterate_average.cpp:205-247).
+__global__ __launch_bounds__(kThreads) void k_decide(DevState* states, const int32_t* __restrict__ pbeg,
+                                                      const int32_t* __restrict__ dbeg,
+                                                      const double* __restrict__ pp,
+                                                      const double* __restrict__ pd,
+                                                      const double* __restrict__ pi) {
+    __shared__ double sh[3 * kWaves];
     __shared__ int stop;
+    const int k = blockIdx.x;
+    DevState* st = states + k;
     if (threadIdx.x == 0) {
         st->avg_pending = 0;
         stop = st->done;
@@ -186,12 +226,12 @@ __global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* st, const d
     __syncthreads();
     if (stop) return;
     double v[3] = {0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < np; i += kReduceThreads) v[0] += pp[i];
-    for (int i = threadIdx.x; i < nd; i += kReduceThreads) {
+    for (int i = pbeg[k] + threadIdx.x; i < pbeg[k + 1]; i += kThreads) v[0] += pp[i];
+    for (int i = dbeg[k] + threadIdx.x; i < dbeg[k + 1]; i += kThreads) {
         v[1] += pd[i];
         v[2] += pi[i];
     }
-    group_sums<kReduceThreads, 3>(v, sh);
+    group_sums<kThreads, 3>(v, sh);
     if (threadIdx.x != 0) return;
     const double pm = v[0], dm = v[1], inter = v[2];
     st->pm = pm;
@@ -210,9 +250,9 @@ __global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* st, const d
         const double cpl = fabs(inter);
         double limit = INFINITY;
         if (cpl > 0.0 && mov > 0.0) limit = mov / (2.0 * cpl);
-        const double k = static_cast<double>(st->iteration) + 2.0;
-        const double shrink = 1.0 - pow(k, -0.3);
-        const double grow = 1.0 + pow(k, -0.6);
+        const double kk = static_cast<double>(st->iteration) + 2.0;
+        const double shrink = 1.0 - pow(kk, -0.3);
+        const double grow = 1.0 + pow(kk, -0.6);
         ok = st->eta <= limit;
         double next = smin(shrink * limit, grow * st->eta);
         next = dclamp(next, st->min_step, 1e12);
@@ -245,25 +285,22 @@ __global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* st, const d
 
 // avg += f * (x - avg) over the freshly committed iterate (iterate_average.cpp:14-24).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict__ st, Iterates<T> it,
-                                                      int n, int m) {
-    if (!st->avg_pending) return;
-    const T f = static_cast<T>(st->avg_frac);
-    const int p = st->parity;
-    const int stride = gridDim.x * kThreads;
-    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
-        if (q < n) {
-            const T a = it.avg_x[q];
-            it.avg_x[q] = a + f * (it.x[p][q] - a);
-        } else {
-            const int i = q - n;
-            const T a = it.avg_y[i];
-            it.avg_y[i] = a + f * (it.y[p][i] - a);
-        }
+__global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict__ states, Iterates<T> it,
+                                                      const Seg* __restrict__ segs, int ncs) {
+    const Seg s = segs[blockIdx.x];
+    const DevState& st = states[s.inst];
+    if (!st.avg_pending || st.phase != PhaseOf<T>::value) return;
+    const T f = static_cast<T>(st.avg_frac);
+    const int p = st.parity;
+    T* __restrict__ avg = static_cast<int>(blockIdx.x) < ncs ? it.avg_x : it.avg_y;
+    const T* __restrict__ cur = static_cast<int>(blockIdx.x) < ncs ? it.x[p] : it.y[p];
+    for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) {
+        const T a = avg[q];
+        avg[q] = a + f * (cur[q] - a);
     }
 }
 
-// Plain row sums out = M v (refreshActivity, debug products).
+// Plain row sums out = M v (debug products, power iteration).
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_spmv(Orient<T> M, const T* __restrict__ v, T* __restrict__ out) {
     __shared__ T lds[kTileNnz];
@@ -272,42 +309,144 @@ __global__ __launch_bounds__(kThreads) void k_spmv(Orient<T> M, const T* __restr
     row_block(M, v, lds, shT, epi);
 }
 
-// Working -> original coordinates: xo = cs * x, yo = rs * y (scaling.cpp:23-37).
+// A x for the current iterate of the instances with flag != 0
+// (refreshActivity, pdhg_kernel.cpp:40-42).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_to_original(int n, int m, const double* __restrict__ cs,
-                                                          const double* __restrict__ rs,
-                                                          const T* __restrict__ x, const T* __restrict__ y,
-                                                          double* __restrict__ xo, double* __restrict__ yo) {
-    const int stride = gridDim.x * kThreads;
-    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
-        if (q < n) xo[q] = cs[q] * static_cast<double>(x[q]);
-        else yo[q - n] = rs[q - n] * static_cast<double>(y[q - n]);
+__global__ __launch_bounds__(kThreads) void k_refresh(Orient<T> a, Iterates<T> it,
+                                                      const DevState* __restrict__ states,
+                                                      const int32_t* __restrict__ flag) {
+    const int k = a.binst[blockIdx.x];
+    if (!flag[k]) return;
+    __shared__ T lds[kTileNnz];
+    __shared__ T shT[kWaves];
+    const int p = states[k].parity;
+    T* __restrict__ out = it.ax[p];
+    auto epi = [&](int r, T s) { out[r] = s; };
+    row_block(a, it.x[p], lds, shT, epi);
+}
+
+// Working -> original coordinates for the instances with mode[k] != 0:
+// xo0/yo0 = scales * current iterate; xo1/yo1 = scales * average when
+// mode[k] == 2, else a copy of xo0/yo0 (scaling.cpp:23-37).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_to_original(const Seg* __restrict__ segs, int ncs,
+                                                          const double* __restrict__ cs,
+                                                          const double* __restrict__ rs, Iterates<T> it,
+                                                          const DevState* __restrict__ states,
+                                                          const int32_t* __restrict__ mode,
+                                                          double* __restrict__ xo0, double* __restrict__ yo0,
+                                                          double* __restrict__ xo1, double* __restrict__ yo1) {
+    const Seg s = segs[blockIdx.x];
+    const int md = mode[s.inst];
+    if (!md) return;
+    const int p = states[s.inst].parity;
+    const bool col = static_cast<int>(blockIdx.x) < ncs;
+    const double* __restrict__ sc = col ? cs : rs;
+    const T* __restrict__ cur = col ? it.x[p] : it.y[p];
+    const T* __restrict__ avg = col ? it.avg_x : it.avg_y;
+    double* __restrict__ o0 = col ? xo0 : yo0;
+    double* __restrict__ o1 = col ? xo1 : yo1;
+    for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) {
+        const double c = sc[q] * static_cast<double>(cur[q]);
+        o0[q] = c;
+        o1[q] = md == 2 ? sc[q] * static_cast<double>(avg[q]) : c;
     }
 }
 
-// Original -> working coordinates: x = xo / cs, y = yo / rs (scaling.cpp:39-53).
+// Original -> working coordinates into the current buffer (parity 0) for the
+// instances with flag != 0: x = xo / cs, y = yo / rs (scaling.cpp:39-53).
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_to_working(int n, int m, const double* __restrict__ cs,
+__global__ __launch_bounds__(kThreads) void k_to_working(const Seg* __restrict__ segs, int ncs,
+                                                         const double* __restrict__ cs,
                                                          const double* __restrict__ rs,
+                                                         const int32_t* __restrict__ flag,
                                                          const double* __restrict__ xo,
-                                                         const double* __restrict__ yo,
-                                                         T* __restrict__ x, T* __restrict__ y) {
-    const int stride = gridDim.x * kThreads;
-    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
-        if (q < n) x[q] = static_cast<T>(xo[q] / cs[q]);
-        else y[q - n] = static_cast<T>(yo[q - n] / rs[q - n]);
+                                                         const double* __restrict__ yo, Iterates<T> it) {
+    const Seg s = segs[blockIdx.x];
+    if (!flag[s.inst]) return;
+    const bool col = static_cast<int>(blockIdx.x) < ncs;
+    const double* __restrict__ sc = col ? cs : rs;
+    const double* __restrict__ src = col ? xo : yo;
+    T* __restrict__ dst = col ? it.x[0] : it.y[0];
+    for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) dst[q] = static_cast<T>(src[q] / sc[q]);
+}
+
+// Per-instance selection: code 1 -> dst = a, code 2 -> dst = b (best-candidate
+// tracking, pdlp_solver.cpp:354-371).
+__global__ __launch_bounds__(kThreads) void k_select(const Seg* __restrict__ segs, int ncs,
+                                                     const int32_t* __restrict__ code, double* __restrict__ dx,
+                                                     double* __restrict__ dy, const double* __restrict__ ax,
+                                                     const double* __restrict__ ay, const double* __restrict__ bx,
+                                                     const double* __restrict__ by) {
+    const Seg s = segs[blockIdx.x];
+    const int c = code[s.inst];
+    if (!c) return;
+    const bool col = static_cast<int>(blockIdx.x) < ncs;
+    double* __restrict__ d = col ? dx : dy;
+    const double* __restrict__ src = c == 1 ? (col ? ax : ay) : (col ? bx : by);
+    for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) d[q] = src[q];
+}
+
+// Restart (pdlp_solver.cpp:437-476) for the instances with code != 0:
+// code 2 adopts the average as the current iterate; then the anchors take the
+// current iterate and the average is cleared.  Also used with code 3 to only
+// reset anchors and average (start of a phase).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_restart(const Seg* __restrict__ segs, int ncs,
+                                                      const DevState* __restrict__ states,
+                                                      const int32_t* __restrict__ code, Iterates<T> it) {
+    const Seg s = segs[blockIdx.x];
+    const int c = code[s.inst];
+    if (!c) return;
+    const int p = states[s.inst].parity;
+    const bool col = static_cast<int>(blockIdx.x) < ncs;
+    T* __restrict__ cur = col ? it.x[p] : it.y[p];
+    T* __restrict__ avg = col ? it.avg_x : it.avg_y;
+    T* __restrict__ anc = col ? it.anc_x : it.anc_y;
+    for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) {
+        if (c == 2) cur[q] = avg[q];
+        anc[q] = cur[q];
+        avg[q] = T(0);
     }
+}
+
+// Squared displacement from the anchors of the candidate a restart would
+// adopt (code 1 current, 2 average); one partial per segment
+// (pdlp_solver.cpp:81-91,456-459).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_sqdist(const Seg* __restrict__ segs, int ncs,
+                                                     const DevState* __restrict__ states,
+                                                     const int32_t* __restrict__ code, Iterates<T> it,
+                                                     double* __restrict__ part) {
+    __shared__ double sh[kWaves];
+    const Seg s = segs[blockIdx.x];
+    const int c = code[s.inst];
+    double v[1] = {0.0};
+    if (c) {
+        const int p = states[s.inst].parity;
+        const bool col = static_cast<int>(blockIdx.x) < ncs;
+        const T* __restrict__ a = c == 2 ? (col ? it.avg_x : it.avg_y) : (col ? it.x[p] : it.y[p]);
+        const T* __restrict__ b = col ? it.anc_x : it.anc_y;
+        for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) {
+            const double d = static_cast<double>(a[q]) - static_cast<double>(b[q]);
+            v[0] += d * d;
+        }
+    }
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
 // Termination row pass for two candidates (termination.cpp:66-135) on the
 // original matrix: violation^2, support and dual-cone drift^2 per candidate.
-__global__ __launch_bounds__(kThreads) void k_term_rows(Orient<double> a, const double* __restrict__ x0,
+__global__ __launch_bounds__(kThreads) void k_term_rows(Orient<double> a, const int32_t* __restrict__ mode,
+                                                        const double* __restrict__ x0,
                                                         const double* __restrict__ x1,
                                                         const double* __restrict__ y0,
                                                         const double* __restrict__ y1,
                                                         const double* __restrict__ lo,
                                                         const double* __restrict__ hi,
                                                         double* __restrict__ part) {
+    if (!mode[a.binst[blockIdx.x]]) return;
     __shared__ double lds[2 * kTileNnz];
     __shared__ double shT[kWaves];
     __shared__ double sh[6 * kWaves];
@@ -336,7 +475,8 @@ __global__ __launch_bounds__(kThreads) void k_term_rows(Orient<double> a, const 
 
 // Termination column pass for two candidates (termination.cpp:140-209):
 // c^T x, the reduced-cost infimum over the box and dual drift^2.
-__global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const double* __restrict__ y0,
+__global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const int32_t* __restrict__ mode,
+                                                        const double* __restrict__ y0,
                                                         const double* __restrict__ y1,
                                                         const double* __restrict__ x0,
                                                         const double* __restrict__ x1,
@@ -344,6 +484,7 @@ __global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const
                                                         const double* __restrict__ lo,
                                                         const double* __restrict__ hi,
                                                         double* __restrict__ part) {
+    if (!mode[at.binst[blockIdx.x]]) return;
     __shared__ double lds[2 * kTileNnz];
     __shared__ double shT[kWaves];
     __shared__ double sh[6 * kWaves];
@@ -370,21 +511,174 @@ __global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const
         for (int q = 0; q < 6; ++q) part[q * at.nblk + blockIdx.x] = acc[q];
 }
 
-// Fixed-order reduction of up to 16 partial arrays, one workgroup each.
-struct ReduceJob {
+// Segmented fixed-order reduction: for quantity q and instance k,
+// dst[q][k] = sum of src[q][beg_q[k] .. beg_q[k+1]).  Grid (B, Q).
+struct SegReduce {
     const double* src[16];
-    int32_t count[16];
+    const int32_t* beg[16];
     double* dst[16];
 };
 
-__global__ __launch_bounds__(kReduceThreads) void k_reduce(ReduceJob job) {
-    __shared__ double sh[kReduceThreads / 64];
-    const int q = blockIdx.x;
+__global__ __launch_bounds__(kThreads) void k_reduce_seg(SegReduce job) {
+    __shared__ double sh[kWaves];
+    const int k = blockIdx.x;
+    const int q = blockIdx.y;
     const double* __restrict__ s = job.src[q];
     double v[1] = {0.0};
-    for (int i = threadIdx.x; i < job.count[q]; i += kReduceThreads) v[0] += s[i];
-    group_sums<kReduceThreads, 1>(v, sh);
-    if (threadIdx.x == 0) *job.dst[q] = v[0];
+    for (int i = job.beg[q][k] + threadIdx.x; i < job.beg[q][k + 1]; i += kThreads) v[0] += s[i];
+    group_sums<kThreads, 1>(v, sh);
+    if (threadIdx.x == 0) job.dst[q][k] = v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Ray certificates (infeasibility.cpp:51-192) for the instances with mode != 0.
+// ---------------------------------------------------------------------------
+// Directions in original coordinates: rx = cs * (x - anchor_x),
+// ry = rs * (y - anchor_y) (pdlp_solver.cpp:378-390); squared norms per segment.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_ray_dir(const Seg* __restrict__ segs, int ncs,
+                                                      const DevState* __restrict__ states,
+                                                      const int32_t* __restrict__ mode,
+                                                      const double* __restrict__ cs,
+                                                      const double* __restrict__ rs, Iterates<T> it,
+                                                      double* __restrict__ rx, double* __restrict__ ry,
+                                                      double* __restrict__ part) {
+    __shared__ double sh[kWaves];
+    const Seg s = segs[blockIdx.x];
+    double v[1] = {0.0};
+    if (mode[s.inst]) {
+        const int p = states[s.inst].parity;
+        const bool col = static_cast<int>(blockIdx.x) < ncs;
+        const double* __restrict__ sc = col ? cs : rs;
+        const T* __restrict__ cur = col ? it.x[p] : it.y[p];
+        const T* __restrict__ anc = col ? it.anc_x : it.anc_y;
+        double* __restrict__ out = col ? rx : ry;
+        for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) {
+            const double d = sc[q] * (static_cast<double>(cur[q]) - static_cast<double>(anc[q]));
+            out[q] = d;
+            v[0] += d * d;
+        }
+    }
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Dual ray, row terms (infeasibility.cpp:92-115): u = ry / ||ry||, drift and
+// support over the row bounds.  ny2[k] = ||ry||^2 of instance k.
+__global__ __launch_bounds__(kThreads) void k_farkas_rows(const Seg* __restrict__ segs, int ncs,
+                                                          const int32_t* __restrict__ mode,
+                                                          const double* __restrict__ ry,
+                                                          const double* __restrict__ ny2,
+                                                          const double* __restrict__ lo,
+                                                          const double* __restrict__ hi, double* __restrict__ u,
+                                                          double* __restrict__ part, int nrs) {
+    __shared__ double sh[2 * kWaves];
+    const Seg s = segs[ncs + blockIdx.x];
+    double v[2] = {0.0, 0.0};   // violation^2, support
+    if (mode[s.inst]) {
+        const double inv = 1.0 / sqrt(ny2[s.inst]);
+        for (int i = s.begin + threadIdx.x; i < s.end; i += kThreads) {
+            const double val = ry[i] * inv;
+            u[i] = val;
+            double q = val;
+            if (!isfinite(hi[i])) q = smin(q, 0.0);
+            if (!isfinite(lo[i])) q = smax(q, 0.0);
+            const double dr = val - q;
+            v[0] += dr * dr;
+            if (q > 0.0) v[1] += q * hi[i];
+            else if (q < 0.0) v[1] += q * lo[i];
+        }
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[nrs + blockIdx.x] = v[1];
+    }
+}
+
+// Dual ray, column terms (infeasibility.cpp:62-90): A^T u, its cone drift and
+// the box infimum.
+__global__ __launch_bounds__(kThreads) void k_farkas_cols(Orient<double> at, const int32_t* __restrict__ mode,
+                                                          const double* __restrict__ u,
+                                                          const double* __restrict__ lo,
+                                                          const double* __restrict__ hi,
+                                                          double* __restrict__ part) {
+    if (!mode[at.binst[blockIdx.x]]) return;
+    __shared__ double lds[kTileNnz];
+    __shared__ double shT[kWaves];
+    __shared__ double sh[2 * kWaves];
+    double v[2] = {0.0, 0.0};   // violation^2, infimum
+    auto epi = [&](int j, double rc) {
+        double q = rc;
+        if (!isfinite(lo[j])) q = smin(q, 0.0);
+        if (!isfinite(hi[j])) q = smax(q, 0.0);
+        const double dr = rc - q;
+        v[0] += dr * dr;
+        if (q > 0.0) v[1] += q * lo[j];
+        else if (q < 0.0) v[1] += q * hi[j];
+    };
+    row_block(at, u, lds, shT, epi);
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[at.nblk + blockIdx.x] = v[1];
+    }
+}
+
+// Primal ray, column terms (infeasibility.cpp:142-161): u = rx / ||rx||
+// projected onto the box's recession cone; drift and c^T d.
+__global__ __launch_bounds__(kThreads) void k_ray_cols(const Seg* __restrict__ segs,
+                                                       const int32_t* __restrict__ mode,
+                                                       const double* __restrict__ rx,
+                                                       const double* __restrict__ nx2,
+                                                       const double* __restrict__ c,
+                                                       const double* __restrict__ lo,
+                                                       const double* __restrict__ hi, double* __restrict__ proj,
+                                                       double* __restrict__ part, int ncs) {
+    __shared__ double sh[2 * kWaves];
+    const Seg s = segs[blockIdx.x];
+    double v[2] = {0.0, 0.0};   // violation^2, objective rate
+    if (mode[s.inst]) {
+        const double inv = 1.0 / sqrt(nx2[s.inst]);
+        for (int j = s.begin + threadIdx.x; j < s.end; j += kThreads) {
+            const double val = rx[j] * inv;
+            double q = val;
+            if (isfinite(hi[j])) q = smin(q, 0.0);
+            if (isfinite(lo[j])) q = smax(q, 0.0);
+            const double dr = val - q;
+            v[0] += dr * dr;
+            proj[j] = q;
+            v[1] += c[j] * q;
+        }
+    }
+    block_sums<2>(v, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v[0];
+        part[ncs + blockIdx.x] = v[1];
+    }
+}
+
+// Primal ray, row terms (infeasibility.cpp:163-178): A d inside the row
+// bounds' recession cone.
+__global__ __launch_bounds__(kThreads) void k_ray_rows(Orient<double> a, const int32_t* __restrict__ mode,
+                                                       const double* __restrict__ proj,
+                                                       const double* __restrict__ lo,
+                                                       const double* __restrict__ hi, double* __restrict__ part) {
+    if (!mode[a.binst[blockIdx.x]]) return;
+    __shared__ double lds[kTileNnz];
+    __shared__ double shT[kWaves];
+    __shared__ double sh[kWaves];
+    double v[1] = {0.0};
+    auto epi = [&](int i, double act) {
+        double q = act;
+        if (isfinite(hi[i])) q = smin(q, 0.0);
+        if (isfinite(lo[i])) q = smax(q, 0.0);
+        const double dr = act - q;
+        v[0] += dr * dr;
+    };
+    row_block(a, proj, lds, shT, epi);
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -392,7 +686,8 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(ReduceJob job) {
 // (preconditioner.cpp:31-80), start point (pdlp_solver.cpp:30-41,168-176).
 // The maxima are order independent and the sums run in the reference's order
 // (CSR order for rows, CSC = increasing-row order for columns), so the scale
-// vectors are bitwise those of the reference.
+// vectors are bitwise those of the reference.  Rows of different instances
+// never interact, so these run over the whole batch unchanged.
 // ---------------------------------------------------------------------------
 __global__ void k_ruiz_max(int nrows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
                            const double* __restrict__ val, const double* __restrict__ own,
@@ -433,14 +728,12 @@ __global__ void k_scale_values(int nrows, const int32_t* __restrict__ ptr, const
 
 // Sum of |a| along each row of one orientation -> step scale 1/sum (or 1).
 __global__ void k_abs_sums(int nrows, const int32_t* __restrict__ ptr, const double* __restrict__ val,
-                           double* __restrict__ scale, double* __restrict__ inv) {
+                           double* __restrict__ scale) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrows) return;
     double acc = 0.0;
     for (int k = ptr[r]; k < ptr[r + 1]; ++k) acc += fabs(val[k]);
-    const double s = acc > 1e-12 ? 1.0 / acc : 1.0;
-    scale[r] = s;
-    inv[r] = 1.0 / s;
+    scale[r] = acc > 1e-12 ? 1.0 / acc : 1.0;
 }
 
 __device__ inline double scale_bound(double b, double f) { return isfinite(b) ? b * f : b; }
@@ -481,214 +774,61 @@ __global__ void k_fill(int64_t n, T* __restrict__ dst, T value) {
 }
 
 // Start point in working coordinates (pdlp_solver.cpp:30-41): x from the
-// bounds, y = 0; or, with a warm start (original coordinates), the warm point
-// mapped to working coordinates and projected into the box / dual cone.
-__global__ void k_start(int n, int m, const double* __restrict__ lo, const double* __restrict__ hi,
-                        const double* __restrict__ rlo, const double* __restrict__ rhi,
-                        const double* __restrict__ cs, const double* __restrict__ rs,
-                        const double* __restrict__ wx, const double* __restrict__ wy,
-                        double* __restrict__ x, double* __restrict__ y) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) {
-        const double l = lo[q], u = hi[q];
-        if (wx) {
-            x[q] = clip(wx[q] / cs[q], l, u);
-        } else {
+// bounds, y = 0; where the instance has a warm start (has_wx / has_wy), the
+// warm point (original coordinates) mapped to working coordinates and
+// projected into the box / dual cone.
+__global__ void k_start(const Seg* __restrict__ segs, int ncs, const double* __restrict__ lo,
+                        const double* __restrict__ hi, const double* __restrict__ rlo,
+                        const double* __restrict__ rhi, const double* __restrict__ cs,
+                        const double* __restrict__ rs, const int32_t* __restrict__ has_wx,
+                        const int32_t* __restrict__ has_wy, const double* __restrict__ wx,
+                        const double* __restrict__ wy, double* __restrict__ x, double* __restrict__ y) {
+    const Seg s = segs[blockIdx.x];
+    if (static_cast<int>(blockIdx.x) < ncs) {
+        const bool warm = has_wx[s.inst] != 0;
+        for (int q = s.begin + threadIdx.x; q < s.end; q += kThreads) {
+            const double l = lo[q], u = hi[q];
+            if (warm) {
+                x[q] = clip(wx[q] / cs[q], l, u);
+            } else {
+                double v = 0.0;
+                if (l <= 0.0 && 0.0 <= u) v = 0.0;
+                else if (isfinite(l)) v = l;
+                else if (isfinite(u)) v = u;
+                x[q] = v;
+            }
+        }
+    } else {
+        const bool warm = has_wy[s.inst] != 0;
+        for (int i = s.begin + threadIdx.x; i < s.end; i += kThreads) {
             double v = 0.0;
-            if (l <= 0.0 && 0.0 <= u) v = 0.0;
-            else if (isfinite(l)) v = l;
-            else if (isfinite(u)) v = u;
-            x[q] = v;
-        }
-    } else if (q < n + m) {
-        const int i = q - n;
-        double v = 0.0;
-        if (wy) {
-            v = wy[i] / rs[i];
-            if (!isfinite(rhi[i])) v = smin(v, 0.0);
-            if (!isfinite(rlo[i])) v = smax(v, 0.0);
-        }
-        y[i] = v;
-    }
-}
-
-// Streaming partial sums: sum (a-b)^2 over two vector pairs (restart
-// displacement, pdlp_solver.cpp:81-91,456-459).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_sqdist(int n, const T* __restrict__ a0, const T* __restrict__ b0,
-                                                     int m, const T* __restrict__ a1, const T* __restrict__ b1,
-                                                     double* __restrict__ part) {
-    __shared__ double sh[2 * kWaves];
-    double v[2] = {0.0, 0.0};
-    const int stride = gridDim.x * kThreads;
-    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
-        if (q < n) {
-            const double d = static_cast<double>(a0[q]) - static_cast<double>(b0[q]);
-            v[0] += d * d;
-        } else {
-            const double d = static_cast<double>(a1[q - n]) - static_cast<double>(b1[q - n]);
-            v[1] += d * d;
+            if (warm) {
+                v = wy[i] / rs[i];
+                if (!isfinite(rhi[i])) v = smin(v, 0.0);
+                if (!isfinite(rlo[i])) v = smax(v, 0.0);
+            }
+            y[i] = v;
         }
     }
-    block_sums<2>(v, sh);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = v[0];
-        part[gridDim.x + blockIdx.x] = v[1];
-    }
 }
 
-// Ray directions in original coordinates: rx = cs * (x - anchor_x),
-// ry = rs * (y - anchor_y) (pdlp_solver.cpp:378-390), with their squared norms.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_ray_dir(int n, int m, const double* __restrict__ cs,
-                                                      const double* __restrict__ rs, const T* __restrict__ x,
-                                                      const T* __restrict__ ax, const T* __restrict__ y,
-                                                      const T* __restrict__ ay, double* __restrict__ rx,
-                                                      double* __restrict__ ry, double* __restrict__ part) {
-    __shared__ double sh[2 * kWaves];
-    double v[2] = {0.0, 0.0};
-    const int stride = gridDim.x * kThreads;
-    for (int q = blockIdx.x * kThreads + threadIdx.x; q < n + m; q += stride) {
-        if (q < n) {
-            const double d = cs[q] * (static_cast<double>(x[q]) - static_cast<double>(ax[q]));
-            rx[q] = d;
-            v[0] += d * d;
-        } else {
-            const int i = q - n;
-            const double d = rs[i] * (static_cast<double>(y[i]) - static_cast<double>(ay[i]));
-            ry[i] = d;
-            v[1] += d * d;
-        }
-    }
-    block_sums<2>(v, sh);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = v[0];
-        part[gridDim.x + blockIdx.x] = v[1];
-    }
-}
-
-// Dual ray, row terms (infeasibility.cpp:92-115): u = ry / ||ry||, drift and
-// support over the row bounds.  norms[1] holds ||ry||^2.
-__global__ __launch_bounds__(kThreads) void k_farkas_rows(int m, const double* __restrict__ ry,
-                                                          const double* __restrict__ norms,
-                                                          const double* __restrict__ lo,
-                                                          const double* __restrict__ hi,
-                                                          double* __restrict__ u, double* __restrict__ part) {
-    __shared__ double sh[2 * kWaves];
-    const double inv = 1.0 / sqrt(norms[1]);
-    double v[2] = {0.0, 0.0};   // violation^2, support
-    const int stride = gridDim.x * kThreads;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < m; i += stride) {
-        const double val = ry[i] * inv;
-        u[i] = val;
-        double q = val;
-        if (!isfinite(hi[i])) q = smin(q, 0.0);
-        if (!isfinite(lo[i])) q = smax(q, 0.0);
-        const double dr = val - q;
-        v[0] += dr * dr;
-        if (q > 0.0) v[1] += q * hi[i];
-        else if (q < 0.0) v[1] += q * lo[i];
-    }
-    block_sums<2>(v, sh);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = v[0];
-        part[gridDim.x + blockIdx.x] = v[1];
-    }
-}
-
-// Dual ray, column terms (infeasibility.cpp:62-90): A^T u, its cone drift and
-// the box infimum.
-__global__ __launch_bounds__(kThreads) void k_farkas_cols(Orient<double> at, const double* __restrict__ u,
-                                                          const double* __restrict__ lo,
-                                                          const double* __restrict__ hi,
-                                                          double* __restrict__ part) {
-    __shared__ double lds[kTileNnz];
-    __shared__ double shT[kWaves];
-    __shared__ double sh[2 * kWaves];
-    double v[2] = {0.0, 0.0};   // violation^2, infimum
-    auto epi = [&](int j, double rc) {
-        double q = rc;
-        if (!isfinite(lo[j])) q = smin(q, 0.0);
-        if (!isfinite(hi[j])) q = smax(q, 0.0);
-        const double dr = rc - q;
-        v[0] += dr * dr;
-        if (q > 0.0) v[1] += q * lo[j];
-        else if (q < 0.0) v[1] += q * hi[j];
-    };
-    row_block(at, u, lds, shT, epi);
-    block_sums<2>(v, sh);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = v[0];
-        part[at.nblk + blockIdx.x] = v[1];
-    }
-}
-
-// Primal ray, column terms (infeasibility.cpp:142-161): u = rx / ||rx||
-// projected onto the box's recession cone; drift and c^T d.  norms[0] holds
-// ||rx||^2.
-__global__ __launch_bounds__(kThreads) void k_ray_cols(int n, const double* __restrict__ rx,
-                                                       const double* __restrict__ norms,
-                                                       const double* __restrict__ c,
-                                                       const double* __restrict__ lo,
-                                                       const double* __restrict__ hi,
-                                                       double* __restrict__ proj, double* __restrict__ part) {
-    __shared__ double sh[2 * kWaves];
-    const double inv = 1.0 / sqrt(norms[0]);
-    double v[2] = {0.0, 0.0};   // violation^2, objective rate
-    const int stride = gridDim.x * kThreads;
-    for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += stride) {
-        const double val = rx[j] * inv;
-        double q = val;
-        if (isfinite(hi[j])) q = smin(q, 0.0);
-        if (isfinite(lo[j])) q = smax(q, 0.0);
-        const double dr = val - q;
-        v[0] += dr * dr;
-        proj[j] = q;
-        v[1] += c[j] * q;
-    }
-    block_sums<2>(v, sh);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = v[0];
-        part[gridDim.x + blockIdx.x] = v[1];
-    }
-}
-
-// Primal ray, row terms (infeasibility.cpp:163-178): A d inside the row
-// bounds' recession cone.
-__global__ __launch_bounds__(kThreads) void k_ray_rows(Orient<double> a, const double* __restrict__ proj,
-                                                       const double* __restrict__ lo,
-                                                       const double* __restrict__ hi,
-                                                       double* __restrict__ part) {
-    __shared__ double lds[kTileNnz];
-    __shared__ double shT[kWaves];
+// Power iteration helpers (preconditioner.cpp:82-140): per-segment dot
+// products over the column segments, and per-instance rescaling.
+__global__ __launch_bounds__(kThreads) void k_dot_seg(const Seg* __restrict__ segs, const double* __restrict__ a,
+                                                      const double* __restrict__ b, double* __restrict__ part) {
     __shared__ double sh[kWaves];
+    const Seg s = segs[blockIdx.x];
     double v[1] = {0.0};
-    auto epi = [&](int i, double act) {
-        double q = act;
-        if (isfinite(hi[i])) q = smin(q, 0.0);
-        if (isfinite(lo[i])) q = smax(q, 0.0);
-        const double dr = act - q;
-        v[0] += dr * dr;
-    };
-    row_block(a, proj, lds, shT, epi);
+    for (int j = s.begin + threadIdx.x; j < s.end; j += kThreads) v[0] += a[j] * b[j];
     block_sums<1>(v, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
 
-// Power iteration helpers (preconditioner.cpp:82-140): dot product partials.
-__global__ __launch_bounds__(kThreads) void k_dot(int n, const double* __restrict__ a,
-                                                  const double* __restrict__ b, double* __restrict__ part) {
-    __shared__ double sh[kWaves];
-    double v[1] = {0.0};
-    const int stride = gridDim.x * kThreads;
-    for (int j = blockIdx.x * kThreads + threadIdx.x; j < n; j += stride) v[0] += a[j] * b[j];
-    block_sums<1>(v, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
-}
-
-__global__ void k_scale_vec(int n, double* __restrict__ v, double s) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) v[j] *= s;
+__global__ __launch_bounds__(kThreads) void k_scale_seg(const Seg* __restrict__ segs, double* __restrict__ v,
+                                                        const double* __restrict__ factor) {
+    const Seg s = segs[blockIdx.x];
+    const double f = factor[s.inst];
+    for (int j = s.begin + threadIdx.x; j < s.end; j += kThreads) v[j] *= f;
 }
 
 }  // namespace pdlpx_dev

@@ -94,6 +94,39 @@ class Engine:
         self._check(rc, "pdlpx_solve")
         return _result(s, x, y, ray, lp)
 
+    def load_batch(self, lps: list[Lp], precision: int = _native.FP64) -> None:
+        """Loads several LPs as one device batch (solved together by solve_batch)."""
+        self._batch = list(lps)
+        self._probs = (_native.Problem * len(lps))(*[lp.problem() for lp in lps])
+        self._check(self.lib.pdlpx_load_batch(self.ctx, len(lps), self._probs, precision), "pdlpx_load_batch")
+        self.lp = None
+
+    def solve_batch(self, options: _native.Options | None = None, warm_x=None, warm_y=None) -> list[PdlpResult]:
+        """Solves every LP of the loaded batch in one batched device run."""
+        lps = self._batch
+        B = len(lps)
+        o = options if options is not None else default_options()
+        xs = [np.zeros(lp.n) for lp in lps]
+        ys = [np.zeros(lp.m) for lp in lps]
+        P = C.c_void_p * B
+        xp = P(*[x.ctypes.data for x in xs])
+        yp = P(*[y.ctypes.data for y in ys])
+
+        def ptrs(vs):
+            if vs is None:
+                return None, []
+            keep = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in vs]
+            return P(*[None if v is None else v.ctypes.data for v in keep]), keep
+
+        wxp, kx = ptrs(warm_x)
+        wyp, ky = ptrs(warm_y)
+        sums = (_native.Summary * B)()
+        self._check(self.lib.pdlpx_solve_loaded(self.ctx, C.byref(o), wxp, wyp, sums, xp, yp),
+                    "pdlpx_solve_loaded")
+        del kx, ky
+        empty = np.zeros(1)
+        return [_result(sums[k], xs[k], ys[k], empty, lps[k]) for k in range(B)]
+
     def spmv(self, v: np.ndarray, transpose: bool = False) -> np.ndarray:
         lp = self.lp
         v = np.ascontiguousarray(v, dtype=np.float64)
