@@ -87,9 +87,23 @@ def main() -> int:
     ap.add_argument("--cpu-sample-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch trial kernels eagerly instead of through the captured hipGraph")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run device copies of MB megabytes (calibrates rocprofv3 FETCH/WRITE counters)")
     ap.add_argument("--iteration-limit", type=int, default=10_000_000,
                     help="cap on PDHG iterations (profiling runs; a capped run is not a time-to-tolerance)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        a = torch.rand(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 10}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
               "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
@@ -106,7 +120,7 @@ def main() -> int:
     fp32 = args.config == 2
     prec = _native.FP32 if fp32 else _native.FP64
     o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=prec,
-                        iteration_limit=args.iteration_limit)
+                        iteration_limit=args.iteration_limit, eager_launch=int(args.eager))
 
     eng = Engine(0)
     eng.load(lp, prec)

@@ -120,7 +120,9 @@ typedef struct pdlpx_options {
                                              best KKT score <= factor * tolerance */
     int32_t fp32_stall_checks;            /* ... or after this many checks without
                                              improving the best score */
-    int32_t reserved;
+    int32_t eager_launch;                 /* 1: launch the trial kernels directly on the
+                                             stream instead of replaying a captured
+                                             hipGraph (same kernels, same order) */
 } pdlpx_options;
 
 typedef struct pdlpx_summary {

@@ -52,7 +52,7 @@ class Options(C.Structure):
         ("use_feasibility_polishing", i32), ("minimum_restart_iterations", i32),
         ("maximum_restart_iterations", i32), ("polishing_iterations", i32),
         ("precision", i32), ("fp32_handoff_factor", f64), ("fp32_stall_checks", i32),
-        ("reserved", i32),
+        ("eager_launch", i32),
     ]
 
 

@@ -3,25 +3,37 @@
 // Row-block SpMV: the host cuts the rows of one matrix orientation (CSR rows
 // of A, or CSC columns of A = rows of A^T) into blocks of at most kTileNnz
 // nonzeros and kTileRows rows; a row longer than kTileNnz gets a block of its
-// own.  One 256-thread workgroup per block streams the block's nonzeros with
-// fully coalesced loads, multiplies by the gathered vector entry and stages
-// the products in LDS; then one thread per row sums its contiguous LDS slice
-// in order and hands the finished row to a fused epilogue (prox step,
-// residual terms, ...).  Long rows are reduced across the whole workgroup.
-// Per-block partial reductions are combined by a separate fixed-order pass, so
-// every result is run-to-run deterministic.
+// own.  One 256-thread workgroup per block:
+//   1. every thread issues, at once, the loads of its row's epilogue data and
+//      of its (at most kPerThread) nonzeros' indices and values -- all
+//      independent, so they are in flight together;
+//   2. then all of its gathers of the vector entries;
+//   3. stages the products in LDS; after one barrier a group of lanes sums
+//      each row's contiguous LDS slice and its first lane runs the fused
+//      epilogue.
+// Long rows stream through the whole workgroup in chunks of kTileNnz and are
+// reduced across it.  Per-block partial reductions are combined by a separate
+// fixed-order pass, so every result is run-to-run deterministic.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#ifndef PDLPX_TILE_NNZ
+#define PDLPX_TILE_NNZ 2048
+#endif
+#ifndef PDLPX_NT_LOADS
+#define PDLPX_NT_LOADS 0
+#endif
+
 namespace pdlpx_dev {
 
-constexpr int kThreads = 256;          // workgroup size of every row-block kernel
+constexpr int kThreads = 256;                    // workgroup size of every row-block kernel
 constexpr int kWaves = kThreads / 64;
-constexpr int kTileNnz = 1024;         // nonzeros staged per block
-constexpr int kTileRows = kThreads;    // rows per block (one thread each)
-constexpr int kReduceThreads = 1024;   // single-workgroup final reductions
+constexpr int kTileNnz = PDLPX_TILE_NNZ;         // nonzeros staged per block
+constexpr int kPerThread = kTileNnz / kThreads;  // nonzeros per thread per chunk
+constexpr int kTileRows = kThreads;              // rows per block (one thread each)
+constexpr int kReduceThreads = 1024;             // single-workgroup final reductions
 
 // Matrix orientation as the kernels see it: rows of this orientation.
 template <typename T>
@@ -34,6 +46,17 @@ struct Orient {
     int32_t nblk;
     int32_t nrows;
 };
+
+// Streamed (read-once-per-pass) data.  PDLPX_NT_LOADS=1 marks it
+// non-temporal; measured slower on gfx950 for these kernels, so off.
+template <typename T>
+__device__ inline T stream_load(const T* p) {
+#if PDLPX_NT_LOADS
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
 
 template <typename T>
 __device__ inline T wave_sum(T v) {
@@ -79,78 +102,193 @@ __device__ inline T block_total(T v, T* sh) {
     return total;
 }
 
-// One row block of y = M v with a per-row epilogue epi(row, sum).  `lds` holds
-// kTileNnz elements of T, `shT` kWaves elements of T.
+// Products of one chunk of up to kTileNnz nonzeros [k0, k1) into lds[0..k1-k0):
+// all index/value loads first, then all gathers.
+template <typename T>
+__device__ inline void stage_chunk(const Orient<T>& M, const T* __restrict__ v, int k0, int k1, T* lds) {
+    const int t = threadIdx.x;
+    int col[kPerThread];
+    T a[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int k = k0 + t + q * kThreads;
+        if (k < k1) {
+            col[q] = stream_load(M.idx + k);
+            a[q] = stream_load(M.val + k);
+        }
+    }
+    T g[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int k = k0 + t + q * kThreads;
+        if (k < k1) g[q] = v[col[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int k = k0 + t + q * kThreads;
+        if (k < k1) lds[k - k0] = a[q] * g[q];
+    }
+}
+
+// Same for two vectors through the same nonzeros.
+template <typename T>
+__device__ inline void stage_chunk2(const Orient<T>& M, const T* __restrict__ v0, const T* __restrict__ v1, int k0,
+                                    int k1, T* lds) {
+    const int t = threadIdx.x;
+    int col[kPerThread];
+    T a[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int k = k0 + t + q * kThreads;
+        if (k < k1) {
+            col[q] = stream_load(M.idx + k);
+            a[q] = stream_load(M.val + k);
+        }
+    }
+    T g0[kPerThread], g1[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int k = k0 + t + q * kThreads;
+        if (k < k1) {
+            g0[q] = v0[col[q]];
+            g1[q] = v1[col[q]];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const int k = k0 + t + q * kThreads;
+        if (k < k1) {
+            lds[k - k0] = a[q] * g0[q];
+            lds[kTileNnz + k - k0] = a[q] * g1[q];
+        }
+    }
+}
+
+// Threads per row for the in-LDS reduction of a block of `rows` rows: the
+// largest power of two (at most 64, one wave) with rows * tpr <= kThreads.
+__device__ inline int threads_per_row(int rows) {
+    int tpr = 1;
+    while (tpr < 64 && 2 * tpr * rows <= kThreads) tpr *= 2;
+    return tpr;
+}
+
+// Sum of a group of `tpr` consecutive lanes (tpr a power of two <= 64),
+// valid in the group's first lane.
+template <typename T>
+__device__ inline T group_sum(T v, int tpr) {
+    for (int off = tpr >> 1; off > 0; off >>= 1) v += __shfl_down(v, off, tpr);
+    return v;
+}
+
+// One row block of y = M v with a fused per-row epilogue.  The epilogue is
+// an object with
+//   Pre load(int row)            -- per-row data, issued before the tile loads
+//   void apply(int row, T sum, const Pre&)
+// `lds` holds kTileNnz elements of T, `shT` kWaves elements of T.  Each row
+// of the block is summed from LDS by a group of threads_per_row(rows) lanes
+// (strided, then a shuffle tree), so a block of few long rows still uses the
+// whole workgroup; the group's first lane runs the epilogue.
 template <typename T, typename Epi>
-__device__ inline void row_block(const Orient<T>& M, const T* __restrict__ v, T* lds, T* shT,
-                                 Epi& epi) {
+__device__ inline void row_block(const Orient<T>& M, const T* __restrict__ v, T* lds, T* shT, Epi& epi) {
     const int b = blockIdx.x;
     const int r0 = M.blk[b];
     const int r1 = M.blk[b + 1];
+    const int t = threadIdx.x;
+    const int tpr = threads_per_row(r1 - r0);
+    const int rr = t / tpr;
+    const int g = t - rr * tpr;
+    const int r = r0 + rr;
+    const bool mine = r < r1;
+    typename Epi::Pre pre{};
+    int a = 0, e = 0;
+    if (mine) {
+        if (g == 0) pre = epi.load(r);
+        a = M.ptr[r];
+        e = M.ptr[r + 1];
+    }
     const int k0 = M.ptr[r0];
     const int k1 = M.ptr[r1];
-    const int t = threadIdx.x;
-    if (k1 - k0 > kTileNnz) {
+    if (k1 - k0 > kTileNnz) {   // one long row: stream it through the workgroup
         T s = T(0);
-        for (int k = k0 + t; k < k1; k += kThreads) s += M.val[k] * v[M.idx[k]];
+        for (int c0 = k0; c0 < k1; c0 += kTileNnz) {
+            const int c1 = c0 + kTileNnz < k1 ? c0 + kTileNnz : k1;
+            int col[kPerThread];
+            T av[kPerThread];
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const int k = c0 + t + q * kThreads;
+                if (k < c1) {
+                    col[q] = stream_load(M.idx + k);
+                    av[q] = stream_load(M.val + k);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kPerThread; ++q) {
+                const int k = c0 + t + q * kThreads;
+                if (k < c1) s += av[q] * v[col[q]];
+            }
+        }
         const T total = block_total(s, shT);
-        if (t == 0) epi(r0, total);
+        if (t == 0) epi.apply(r0, total, pre);
         return;
     }
-    for (int k = k0 + t; k < k1; k += kThreads) lds[k - k0] = M.val[k] * v[M.idx[k]];
+    stage_chunk(M, v, k0, k1, lds);
     __syncthreads();
-    const int r = r0 + t;
-    if (r < r1) {
-        const int a = M.ptr[r] - k0;
-        const int e = M.ptr[r + 1] - k0;
-        T s = T(0);
-        for (int q = a; q < e; ++q) s += lds[q];
-        epi(r, s);
-    }
+    T s = T(0);
+    if (mine)
+        for (int q = a - k0 + g; q < e - k0; q += tpr) s += lds[q];
+    s = group_sum(s, tpr);
+    if (mine && g == 0) epi.apply(r, s, pre);
 }
 
 // Two vectors through the same block (termination checks score the current
 // and the averaged iterate in one matrix pass).  `lds` holds 2 * kTileNnz.
+// The epilogue's apply takes (row, sum0, sum1, pre).
 template <typename T, typename Epi>
-__device__ inline void row_block2(const Orient<T>& M, const T* __restrict__ v0,
-                                  const T* __restrict__ v1, T* lds, T* shT, Epi& epi) {
+__device__ inline void row_block2(const Orient<T>& M, const T* __restrict__ v0, const T* __restrict__ v1, T* lds,
+                                  T* shT, Epi& epi) {
     const int b = blockIdx.x;
     const int r0 = M.blk[b];
     const int r1 = M.blk[b + 1];
+    const int t = threadIdx.x;
+    const int tpr = threads_per_row(r1 - r0);
+    const int rr = t / tpr;
+    const int g = t - rr * tpr;
+    const int r = r0 + rr;
+    const bool mine = r < r1;
+    typename Epi::Pre pre{};
+    int a = 0, e = 0;
+    if (mine) {
+        if (g == 0) pre = epi.load(r);
+        a = M.ptr[r];
+        e = M.ptr[r + 1];
+    }
     const int k0 = M.ptr[r0];
     const int k1 = M.ptr[r1];
-    const int t = threadIdx.x;
     if (k1 - k0 > kTileNnz) {
         T s0 = T(0), s1 = T(0);
         for (int k = k0 + t; k < k1; k += kThreads) {
-            const T a = M.val[k];
-            const int c = M.idx[k];
-            s0 += a * v0[c];
-            s1 += a * v1[c];
+            const T av = stream_load(M.val + k);
+            const int c = stream_load(M.idx + k);
+            s0 += av * v0[c];
+            s1 += av * v1[c];
         }
         const T t0 = block_total(s0, shT);
         const T t1 = block_total(s1, shT);
-        if (t == 0) epi(r0, t0, t1);
+        if (t == 0) epi.apply(r0, t0, t1, pre);
         return;
     }
-    for (int k = k0 + t; k < k1; k += kThreads) {
-        const T a = M.val[k];
-        const int c = M.idx[k];
-        lds[k - k0] = a * v0[c];
-        lds[kTileNnz + k - k0] = a * v1[c];
-    }
+    stage_chunk2(M, v0, v1, k0, k1, lds);
     __syncthreads();
-    const int r = r0 + t;
-    if (r < r1) {
-        const int a = M.ptr[r] - k0;
-        const int e = M.ptr[r + 1] - k0;
-        T s0 = T(0), s1 = T(0);
-        for (int q = a; q < e; ++q) {
+    T s0 = T(0), s1 = T(0);
+    if (mine)
+        for (int q = a - k0 + g; q < e - k0; q += tpr) {
             s0 += lds[q];
             s1 += lds[kTileNnz + q];
         }
-        epi(r, s0, s1);
-    }
+    s0 = group_sum(s0, tpr);
+    s1 = group_sum(s1, tpr);
+    if (mine && g == 0) epi.apply(r, s0, s1, pre);
 }
 
 // std::max / std::min / clip with the reference's argument order, so NaN and

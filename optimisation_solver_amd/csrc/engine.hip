@@ -206,6 +206,10 @@ struct pdlpx_ctx {
     std::vector<int> dev;              // device instance -> caller index
     int B = 0;                         // device instances
     std::vector<int32_t> rowOff, colOff;
+    // Degree-sort permutations (per instance, stable, longest first), global
+    // indices: xNew[j] = internal position of the caller's column j,
+    // xOld[p] = caller column at internal position p; same for rows.
+    std::vector<int32_t> xNew, xOld, yNew, yOld;
     int m = 0, n = 0;
     int64_t nnz = 0;
     OrientBuf A, AT;                   // CSR of A and CSR of A^T (= CSC of A)
@@ -413,7 +417,7 @@ private:
         if (phases & 2) launchPrimal<float>();
         if (phases & 1) launchDual<double>();
         if (phases & 2) launchDual<float>();
-        k_decide<<<B, kThreads, 0, s>>>(X.st.p, X.AT.ibeg.p, X.A.ibeg.p, X.part_p.p, X.part_d.p, X.part_i.p);
+        k_decide<<<B, kReduceThreads, 0, s>>>(X.st.p, X.AT.ibeg.p, X.A.ibeg.p, X.part_p.p, X.part_d.p, X.part_i.p);
         if (phases & 1) launchAverage<double>();
         if (phases & 2) launchAverage<float>();
     }
@@ -449,7 +453,12 @@ private:
         const int big = o.termination_check_frequency + 4;
         while (true) {
             const int slots = static_cast<int>(std::min<int64_t>(big, left + 4 > 16 ? big : 16));
-            HIPCHK(hipGraphLaunch(graphFor(phases, slots), s));
+            if (o.eager_launch) {
+                for (int q = 0; q < slots; ++q) launchSlot(phases);
+                HIPCHK(hipGetLastError());
+            } else {
+                HIPCHK(hipGraphLaunch(graphFor(phases, slots), s));
+            }
             pullState();
             bool any = false;
             left = 0;
@@ -667,6 +676,16 @@ private:
         HIPCHK(hipGetLastError());
     }
 
+    // Copies instance slice [off, off+count) of a device vector to the host in
+    // the caller's order (perm = xNew or yNew).
+    std::vector<double> download(const double* dev, int off, int count, const std::vector<int32_t>& perm) {
+        std::vector<double> tmp(count), out(count);
+        if (count) HIPCHK(hipMemcpyAsync(tmp.data(), dev + off, 8 * size_t(count), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        for (int j = 0; j < count; ++j) out[j] = tmp[perm[off + j] - off];
+        return out;
+    }
+
     void initStart(const double* const* warmX, const double* const* warmY);
     void mainLoop();
     void checkMain(const std::vector<int>& atCheck);
@@ -758,7 +777,7 @@ std::vector<double> Run::powerIteration(std::vector<char>& converged) {
         std::mt19937 gen(1);
         std::uniform_real_distribution<double> dist(-1.0, 1.0);
         double* xs = x.data() + X.colOff[k];
-        for (int j = 0; j < in.n; ++j) xs[j] = dist(gen);
+        for (int j = 0; j < in.n; ++j) x[X.xNew[X.colOff[k] + j]] = dist(gen);   // caller order -> internal
         double sq = 0.0;
         for (int j = 0; j < in.n; ++j) sq += xs[j] * xs[j];
         const double norm = std::sqrt(sq);
@@ -811,18 +830,24 @@ std::vector<double> Run::powerIteration(std::vector<char>& converged) {
 
 void Run::initStart(const double* const* warmX, const double* const* warmY) {
     const int n = X.n, m = X.m;
+    std::vector<double> wx, wy;   // warm starts in internal order
     for (int k = 0; k < B; ++k) {
         const int u = X.dev[k];
         fl[F_WX][k] = warmX && warmX[u] ? 1 : 0;
         fl[F_WY][k] = warmY && warmY[u] ? 1 : 0;
-        if (fl[F_WX][k] && X.info[u].n)
-            HIPCHK(hipMemcpyAsync(X.warmx.p + X.colOff[k], warmX[u], sizeof(double) * X.info[u].n,
-                                  hipMemcpyHostToDevice, s));
-        if (fl[F_WY][k] && X.info[u].m)
-            HIPCHK(hipMemcpyAsync(X.warmy.p + X.rowOff[k], warmY[u], sizeof(double) * X.info[u].m,
-                                  hipMemcpyHostToDevice, s));
+        if (fl[F_WX][k]) {
+            if (wx.empty()) wx.assign(n, 0.0);
+            for (int j = 0; j < X.info[u].n; ++j) wx[X.xNew[X.colOff[k] + j]] = warmX[u][j];
+        }
+        if (fl[F_WY][k]) {
+            if (wy.empty()) wy.assign(m, 0.0);
+            for (int i = 0; i < X.info[u].m; ++i) wy[X.yNew[X.rowOff[k] + i]] = warmY[u][i];
+        }
     }
+    if (!wx.empty()) HIPCHK(hipMemcpyAsync(X.warmx.p, wx.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if (!wy.empty()) HIPCHK(hipMemcpyAsync(X.warmy.p, wy.data(), sizeof(double) * m, hipMemcpyHostToDevice, s));
     pushFlags({F_WX, F_WY});
+    HIPCHK(hipStreamSynchronize(s));   // wx / wy die here
     const IterBuf<double>& D = X.it64;
     k_start<<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.w64.lo.p, X.w64.hi.p, X.w64.rlo.p, X.w64.rhi.p, X.cs.p,
                                          X.rs.p, flagDev(F_WX), flagDev(F_WY), X.warmx.p, X.warmy.p, D.x[0].p,
@@ -966,18 +991,10 @@ void Run::checkMain(const std::vector<int>& atCheck) {
             in.rayKind = in.status;
             in.finishTime = since();
             const InstInfo& ii = X.info[X.dev[k]];
-            std::vector<double>& dst = rays[k];
-            if (ray[k] == 1) {
-                dst.resize(ii.m);
-                if (ii.m)
-                    HIPCHK(hipMemcpyAsync(dst.data(), X.ry.p + X.rowOff[k], 8 * size_t(ii.m), hipMemcpyDeviceToHost, s));
-            } else {
-                dst.resize(ii.n);
-                if (ii.n)
-                    HIPCHK(hipMemcpyAsync(dst.data(), X.rx.p + X.colOff[k], 8 * size_t(ii.n), hipMemcpyDeviceToHost, s));
-            }
+            rays[k] = ray[k] == 1 ? download(X.ry.p, X.rowOff[k], ii.m, X.yNew)
+                                  : download(X.rx.p, X.colOff[k], ii.n, X.xNew);
         }
-        if (anyRay) HIPCHK(hipStreamSynchronize(s));
+        (void)anyRay;
     }
     std::vector<char> handoff(B, 0);
     for (int k : atCheck) {
@@ -1206,15 +1223,18 @@ void Run::polishStage() {
 }
 
 void Run::collect(pdlpx_summary* const* sums, double* const* xOut, double* const* yOut, double* const* rayOut) {
+    std::vector<double> hx(X.n), hy(X.m);
+    if (X.n) HIPCHK(hipMemcpyAsync(hx.data(), X.bestx.p, 8 * size_t(X.n), hipMemcpyDeviceToHost, s));
+    if (X.m) HIPCHK(hipMemcpyAsync(hy.data(), X.besty.p, 8 * size_t(X.m), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     for (int k = 0; k < B; ++k) {
         const int u = X.dev[k];
         const InstInfo& in = X.info[u];
-        if (xOut && xOut[u] && in.n)
-            HIPCHK(hipMemcpyAsync(xOut[u], X.bestx.p + X.colOff[k], 8 * size_t(in.n), hipMemcpyDeviceToHost, s));
-        if (yOut && yOut[u] && in.m)
-            HIPCHK(hipMemcpyAsync(yOut[u], X.besty.p + X.rowOff[k], 8 * size_t(in.m), hipMemcpyDeviceToHost, s));
+        if (xOut && xOut[u])
+            for (int j = 0; j < in.n; ++j) xOut[u][j] = hx[X.xNew[X.colOff[k] + j]];
+        if (yOut && yOut[u])
+            for (int i = 0; i < in.m; ++i) yOut[u][i] = hy[X.yNew[X.rowOff[k] + i]];
     }
-    HIPCHK(hipStreamSynchronize(s));
     for (int k = 0; k < B; ++k) {
         const int u = X.dev[k];
         const Inst& in = I[k];
@@ -1362,31 +1382,68 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     X.n = X.colOff[B];
     X.nnz = total;
     if (B == 0) return;
-    // concatenated CSR / CSC with global indices
-    std::vector<int32_t> rp(X.m + 1), cp(X.n + 1), ci(total), ri(total);
-    std::vector<double> rv(total), cv(total), c(X.n), lo(X.n), hi(X.n), rlo(X.m), rhi(X.m);
-    int64_t zr = 0, zc = 0;
+    // Degree-sorted, concatenated CSR / CSC with global indices.  Sorting each
+    // instance's columns (rows) by their nonzero count, longest first, packs
+    // rows of similar length into the same blocks and the most gathered
+    // vector entries into a compact, cache-friendly range.  Entries keep their
+    // original order inside each row and column.
+    X.xNew.assign(X.n, 0);
+    X.xOld.assign(X.n, 0);
+    X.yNew.assign(X.m, 0);
+    X.yOld.assign(X.m, 0);
+    std::vector<int32_t> order;
+    auto sortSpace = [&](int count, int off, const int64_t* start, std::vector<int32_t>& nw,
+                         std::vector<int32_t>& old) {
+        order.resize(count);
+        for (int j = 0; j < count; ++j) order[j] = j;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return start[a + 1] - start[a] > start[b + 1] - start[b];
+        });
+        for (int q = 0; q < count; ++q) {
+            old[off + q] = off + order[q];
+            nw[off + order[q]] = off + q;
+        }
+    };
     for (int k = 0; k < B; ++k) {
         const pdlpx_problem& p = probs[X.dev[k]];
-        const int R = X.rowOff[k], Cc = X.colOff[k];
-        for (int i = 0; i < p.rows; ++i) rp[R + i] = static_cast<int32_t>(zr + p.csr_row_start[i]);
-        for (int j = 0; j < p.cols; ++j) cp[Cc + j] = static_cast<int32_t>(zc + p.csc_col_start[j]);
-        for (int64_t q = 0; q < p.nnz; ++q) {
-            ci[zr + q] = Cc + p.csr_col[q];
-            rv[zr + q] = p.csr_val[q];
-            ri[zc + q] = R + p.csc_row[q];
-            cv[zc + q] = p.csc_val[q];
-        }
-        zr += p.nnz;
-        zc += p.nnz;
-        std::copy(p.objective, p.objective + p.cols, c.begin() + Cc);
-        std::copy(p.var_lower, p.var_lower + p.cols, lo.begin() + Cc);
-        std::copy(p.var_upper, p.var_upper + p.cols, hi.begin() + Cc);
-        std::copy(p.row_lower, p.row_lower + p.rows, rlo.begin() + R);
-        std::copy(p.row_upper, p.row_upper + p.rows, rhi.begin() + R);
+        sortSpace(p.cols, X.colOff[k], p.csc_col_start, X.xNew, X.xOld);
+        sortSpace(p.rows, X.rowOff[k], p.csr_row_start, X.yNew, X.yOld);
     }
-    rp[X.m] = static_cast<int32_t>(zr);
-    cp[X.n] = static_cast<int32_t>(zc);
+    std::vector<int32_t> rp(X.m + 1), cp(X.n + 1), ci(total), ri(total);
+    std::vector<double> rv(total), cv(total), c(X.n), lo(X.n), hi(X.n), rlo(X.m), rhi(X.m);
+    int64_t z = 0;
+    for (int k = 0; k < B; ++k) {   // CSR rows in internal order
+        const pdlpx_problem& p = probs[X.dev[k]];
+        const int R = X.rowOff[k], Cc = X.colOff[k];
+        for (int q = 0; q < p.rows; ++q) {
+            const int i = X.yOld[R + q] - R;
+            rp[R + q] = static_cast<int32_t>(z);
+            for (int64_t e = p.csr_row_start[i]; e < p.csr_row_start[i + 1]; ++e, ++z) {
+                ci[z] = X.xNew[Cc + p.csr_col[e]];
+                rv[z] = p.csr_val[e];
+            }
+            rlo[R + q] = p.row_lower[i];
+            rhi[R + q] = p.row_upper[i];
+        }
+    }
+    rp[X.m] = static_cast<int32_t>(z);
+    z = 0;
+    for (int k = 0; k < B; ++k) {   // CSC columns in internal order
+        const pdlpx_problem& p = probs[X.dev[k]];
+        const int R = X.rowOff[k], Cc = X.colOff[k];
+        for (int q = 0; q < p.cols; ++q) {
+            const int j = X.xOld[Cc + q] - Cc;
+            cp[Cc + q] = static_cast<int32_t>(z);
+            for (int64_t e = p.csc_col_start[j]; e < p.csc_col_start[j + 1]; ++e, ++z) {
+                ri[z] = X.yNew[R + p.csc_row[e]];
+                cv[z] = p.csc_val[e];
+            }
+            c[Cc + q] = p.objective[j];
+            lo[Cc + q] = p.var_lower[j];
+            hi[Cc + q] = p.var_upper[j];
+        }
+    }
+    cp[X.n] = static_cast<int32_t>(z);
     hipStream_t s = X.stream;
     uploadOrient(X.A, rp, X.rowOff, ci, rv, s);
     uploadOrient(X.AT, cp, X.colOff, ri, cv, s);
@@ -1594,17 +1651,22 @@ int pdlpx_spmv(pdlpx_ctx* X, int32_t transpose, const double* x, double* y) {
         hipStream_t s = X->stream;
         const int nin = transpose ? X->m : X->n;
         const int nout = transpose ? X->n : X->m;
+        const std::vector<int32_t>& inNew = transpose ? X->yNew : X->xNew;
+        const std::vector<int32_t>& outNew = transpose ? X->xNew : X->yNew;
+        std::vector<double> hin(nin), hout(nout);
+        for (int q = 0; q < nin; ++q) hin[inNew[q]] = x[q];
         DBuf<double> din, dout;
         din.alloc(nin);
         dout.alloc(nout);
-        if (nin) HIPCHK(hipMemcpyAsync(din.p, x, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+        if (nin) HIPCHK(hipMemcpyAsync(din.p, hin.data(), sizeof(double) * nin, hipMemcpyHostToDevice, s));
         const OrientBuf& M = transpose ? X->AT : X->A;
         if (nout) {
             k_spmv<double><<<M.nblk, kThreads, 0, s>>>(M.view<double>(M.val.p), din.p, dout.p);
             HIPCHK(hipGetLastError());
-            HIPCHK(hipMemcpyAsync(y, dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
         }
         HIPCHK(hipStreamSynchronize(s));
+        for (int q = 0; q < nout; ++q) y[q] = hout[outNew[q]];
     });
 }
 
@@ -1632,12 +1694,17 @@ int pdlpx_scaling(pdlpx_ctx* X, int32_t ruiz_iterations, int32_t use_pc, double*
         run.setupOnly();
         hipStream_t s = X->stream;
         const int m = X->m, n = X->n;
-        if (row_scale && m) HIPCHK(hipMemcpyAsync(row_scale, X->rs.p, 8 * size_t(m), hipMemcpyDeviceToHost, s));
-        if (col_scale && n) HIPCHK(hipMemcpyAsync(col_scale, X->cs.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
-        if (primal_scale && n)
-            HIPCHK(hipMemcpyAsync(primal_scale, X->w64.tau.p, 8 * size_t(n), hipMemcpyDeviceToHost, s));
-        if (dual_scale && m) HIPCHK(hipMemcpyAsync(dual_scale, X->w64.sig.p, 8 * size_t(m), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        auto fetch = [&](double* out, const double* dev, int count, const std::vector<int32_t>& perm) {
+            if (!out || !count) return;
+            std::vector<double> tmp(count);
+            HIPCHK(hipMemcpyAsync(tmp.data(), dev, 8 * size_t(count), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            for (int q = 0; q < count; ++q) out[q] = tmp[perm[q]];
+        };
+        fetch(row_scale, X->rs.p, m, X->yNew);
+        fetch(col_scale, X->cs.p, n, X->xNew);
+        fetch(primal_scale, X->w64.tau.p, n, X->xNew);
+        fetch(dual_scale, X->w64.sig.p, m, X->yNew);
     });
 }
 

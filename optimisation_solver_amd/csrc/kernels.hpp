@@ -104,6 +104,29 @@ __device__ inline double floor_step<double>() { return 1e-300; }   // pdhg_kerne
 template <>
 __device__ inline float floor_step<float>() { return 1e-37f; }
 
+template <typename T>
+struct PrimalEpi {
+    struct Pre {
+        T c, lo, hi, tau, x;
+    };
+    ColData<T> cd;
+    const T* __restrict__ x;
+    T* __restrict__ xt;
+    T mul;
+    double acc = 0.0;
+    __device__ Pre load(int j) const {
+        return Pre{stream_load(cd.c + j), stream_load(cd.lo + j), stream_load(cd.hi + j), stream_load(cd.tau + j),
+                   stream_load(x + j)};
+    }
+    __device__ void apply(int j, T s, const Pre& p) {
+        const T g = p.c + s;
+        const T now = clip(p.x - mul * p.tau * g, p.lo, p.hi);
+        xt[j] = now;
+        const T d = now - p.x;
+        acc += static_cast<double>(d * d * (T(1) / p.tau));
+    }
+};
+
 // Primal half of a trial (pdhg_kernel.cpp:47-104): per column j of the CSC
 // orientation, g_j = c_j + (A^T y)_j and x'_j = clip(x_j - (eta/omega) T_j g_j);
 // accumulates sum dx_j^2 / T_j.
@@ -117,25 +140,42 @@ __global__ __launch_bounds__(kThreads) void k_primal(Orient<T> at, ColData<T> cd
     __shared__ T shT[kWaves];
     __shared__ double sh[kWaves];
     const int p = st.parity;
-    const T mul = static_cast<T>(st.eta_trial / st.omega);
-    const T* __restrict__ x = it.x[p];
-    const T* __restrict__ y = it.y[p];
-    T* __restrict__ xt = it.x[p ^ 1];
-    double acc = 0.0;
-    auto epi = [&](int j, T s) {
-        const T g = cd.c[j] + s;
-        const T was = x[j];
-        const T tau = cd.tau[j];
-        const T now = clip(was - mul * tau * g, cd.lo[j], cd.hi[j]);
-        xt[j] = now;
-        const T d = now - was;
-        acc += static_cast<double>(d * d * (T(1) / tau));
-    };
-    row_block(at, y, lds, shT, epi);
-    double v[1] = {acc};
+    PrimalEpi<T> epi{cd, it.x[p], it.x[p ^ 1], static_cast<T>(st.eta_trial / st.omega)};
+    row_block(at, it.y[p], lds, shT, epi);
+    double v[1] = {epi.acc};
     block_sums<1>(v, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
 }
+
+template <typename T>
+struct DualEpi {
+    struct Pre {
+        T lo, hi, sig, y, ax;
+    };
+    RowData<T> rd;
+    const T* __restrict__ y;
+    const T* __restrict__ ax;
+    T* __restrict__ yt;
+    T* __restrict__ axt;
+    T mul;
+    double mov = 0.0, inter = 0.0;
+    __device__ Pre load(int i) const {
+        return Pre{stream_load(rd.lo + i), stream_load(rd.hi + i), stream_load(rd.sig + i), stream_load(y + i),
+                   stream_load(ax + i)};
+    }
+    __device__ void apply(int i, T act, const Pre& p) {
+        const T dact = act - p.ax;
+        const T ext = act + dact;
+        const T step = smax(mul * p.sig, floor_step<T>());
+        const T v = p.y + step * ext;
+        const T now = smax(v - step * p.hi, T(0)) + smin(v - step * p.lo, T(0));
+        yt[i] = now;
+        axt[i] = act;
+        const T d = now - p.y;
+        mov += static_cast<double>(d * d * (T(1) / p.sig));
+        inter += static_cast<double>(d) * static_cast<double>(dact);
+    }
+};
 
 // Dual half (pdhg_kernel.cpp:109-187): (A x')_i, the extrapolated activity
 // 2 A x' - A x, and the Moreau-form dual prox
@@ -151,29 +191,9 @@ __global__ __launch_bounds__(kThreads) void k_dual(Orient<T> a, RowData<T> rd, I
     __shared__ T shT[kWaves];
     __shared__ double sh[2 * kWaves];
     const int p = st.parity;
-    const T mul = static_cast<T>(st.eta_trial * st.omega);
-    const T* __restrict__ xt = it.x[p ^ 1];
-    const T* __restrict__ y = it.y[p];
-    const T* __restrict__ ax = it.ax[p];
-    T* __restrict__ yt = it.y[p ^ 1];
-    T* __restrict__ axt = it.ax[p ^ 1];
-    double mov = 0.0, inter = 0.0;
-    auto epi = [&](int i, T act) {
-        const T dact = act - ax[i];
-        const T ext = act + dact;
-        const T sig = rd.sig[i];
-        const T step = smax(mul * sig, floor_step<T>());
-        const T was = y[i];
-        const T v = was + step * ext;
-        const T now = smax(v - step * rd.hi[i], T(0)) + smin(v - step * rd.lo[i], T(0));
-        yt[i] = now;
-        axt[i] = act;
-        const T d = now - was;
-        mov += static_cast<double>(d * d * (T(1) / sig));
-        inter += static_cast<double>(d) * static_cast<double>(dact);
-    };
-    row_block(a, xt, lds, shT, epi);
-    double v[2] = {mov, inter};
+    DualEpi<T> epi{rd, it.y[p], it.ax[p], it.y[p ^ 1], it.ax[p ^ 1], static_cast<T>(st.eta_trial * st.omega)};
+    row_block(a, it.x[p ^ 1], lds, shT, epi);
+    double v[2] = {epi.mov, epi.inter};
     block_sums<2>(v, sh);
     if (threadIdx.x == 0) {
         part_d[blockIdx.x] = v[0];
@@ -209,13 +229,14 @@ __device__ inline double dclamp(double v, double lo, double hi) {   // std::clam
 
 // Linesearch decision and commit, one workgroup per instance
 // (step_controller.cpp:30-68, pdlp_solver.cpp:292-348,
-// iterate_average.cpp:205-247).
-__global__ __launch_bounds__(kThreads) void k_decide(DevState* states, const int32_t* __restrict__ pbeg,
-                                                      const int32_t* __restrict__ dbeg,
-                                                      const double* __restrict__ pp,
-                                                      const double* __restrict__ pd,
-                                                      const double* __restrict__ pi) {
-    __shared__ double sh[3 * kWaves];
+// iterate_average.cpp:205-247).  Eight independent accumulators per thread
+// keep many partial loads in flight.
+__global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* states, const int32_t* __restrict__ pbeg,
+                                                            const int32_t* __restrict__ dbeg,
+                                                            const double* __restrict__ pp,
+                                                            const double* __restrict__ pd,
+                                                            const double* __restrict__ pi) {
+    __shared__ double sh[3 * (kReduceThreads / 64)];
     __shared__ int stop;
     const int k = blockIdx.x;
     DevState* st = states + k;
@@ -225,13 +246,34 @@ __global__ __launch_bounds__(kThreads) void k_decide(DevState* states, const int
     }
     __syncthreads();
     if (stop) return;
-    double v[3] = {0.0, 0.0, 0.0};
-    for (int i = pbeg[k] + threadIdx.x; i < pbeg[k + 1]; i += kThreads) v[0] += pp[i];
-    for (int i = dbeg[k] + threadIdx.x; i < dbeg[k + 1]; i += kThreads) {
-        v[1] += pd[i];
-        v[2] += pi[i];
+    constexpr int U = 8;
+    double ap[U] = {}, ad[U] = {}, ai[U] = {};
+    const int p0 = pbeg[k], p1 = pbeg[k + 1], d0 = dbeg[k], d1 = dbeg[k + 1];
+    for (int i = p0 + threadIdx.x; i < p1; i += U * kReduceThreads) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = i + u * kReduceThreads;
+            if (q < p1) ap[u] += pp[q];
+        }
     }
-    group_sums<kThreads, 3>(v, sh);
+    for (int i = d0 + threadIdx.x; i < d1; i += U * kReduceThreads) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = i + u * kReduceThreads;
+            if (q < d1) {
+                ad[u] += pd[q];
+                ai[u] += pi[q];
+            }
+        }
+    }
+    double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        v[0] += ap[u];
+        v[1] += ad[u];
+        v[2] += ai[u];
+    }
+    group_sums<kReduceThreads, 3>(v, sh);
     if (threadIdx.x != 0) return;
     const double pm = v[0], dm = v[1], inter = v[2];
     st->pm = pm;
@@ -302,10 +344,18 @@ __global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict
 
 // Plain row sums out = M v (debug products, power iteration).
 template <typename T>
+struct StoreEpi {
+    struct Pre {};
+    T* __restrict__ out;
+    __device__ Pre load(int) const { return Pre{}; }
+    __device__ void apply(int r, T s, const Pre&) { out[r] = s; }
+};
+
+template <typename T>
 __global__ __launch_bounds__(kThreads) void k_spmv(Orient<T> M, const T* __restrict__ v, T* __restrict__ out) {
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
-    auto epi = [&](int r, T s) { out[r] = s; };
+    StoreEpi<T> epi{out};
     row_block(M, v, lds, shT, epi);
 }
 
@@ -320,8 +370,7 @@ __global__ __launch_bounds__(kThreads) void k_refresh(Orient<T> a, Iterates<T> i
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     const int p = states[k].parity;
-    T* __restrict__ out = it.ax[p];
-    auto epi = [&](int r, T s) { out[r] = s; };
+    StoreEpi<T> epi{it.ax[p]};
     row_block(a, it.x[p], lds, shT, epi);
 }
 
@@ -438,6 +487,33 @@ __global__ __launch_bounds__(kThreads) void k_sqdist(const Seg* __restrict__ seg
 
 // Termination row pass for two candidates (termination.cpp:66-135) on the
 // original matrix: violation^2, support and dual-cone drift^2 per candidate.
+struct TermRowEpi {
+    struct Pre {
+        double lo, hi, y0, y1;
+    };
+    const double* __restrict__ y0;
+    const double* __restrict__ y1;
+    const double* __restrict__ lo;
+    const double* __restrict__ hi;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    __device__ Pre load(int i) const { return Pre{lo[i], hi[i], y0[i], y1[i]}; }
+    __device__ static void one(double act, double yv, double l, double u, double* o) {
+        const double e = act - clip(act, l, u);
+        o[0] += e * e;
+        double q = yv;
+        if (!isfinite(u)) q = smin(q, 0.0);
+        if (!isfinite(l)) q = smax(q, 0.0);
+        const double dr = yv - q;
+        o[2] += dr * dr;
+        if (q > 0.0) o[1] += q * u;
+        else if (q < 0.0) o[1] += q * l;
+    }
+    __device__ void apply(int, double s0, double s1, const Pre& p) {
+        one(s0, p.y0, p.lo, p.hi, acc);
+        one(s1, p.y1, p.lo, p.hi, acc + 3);
+    }
+};
+
 __global__ __launch_bounds__(kThreads) void k_term_rows(Orient<double> a, const int32_t* __restrict__ mode,
                                                         const double* __restrict__ x0,
                                                         const double* __restrict__ x1,
@@ -450,31 +526,43 @@ __global__ __launch_bounds__(kThreads) void k_term_rows(Orient<double> a, const 
     __shared__ double lds[2 * kTileNnz];
     __shared__ double shT[kWaves];
     __shared__ double sh[6 * kWaves];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    auto one = [&](int i, double act, double yv, double* o) {
-        const double l = lo[i], u = hi[i];
-        const double e = act - clip(act, l, u);
-        o[0] += e * e;
-        double q = yv;
-        if (!isfinite(u)) q = smin(q, 0.0);
-        if (!isfinite(l)) q = smax(q, 0.0);
-        const double dr = yv - q;
-        o[2] += dr * dr;
-        if (q > 0.0) o[1] += q * u;
-        else if (q < 0.0) o[1] += q * l;
-    };
-    auto epi = [&](int i, double s0, double s1) {
-        one(i, s0, y0[i], acc);
-        one(i, s1, y1[i], acc + 3);
-    };
+    TermRowEpi epi{y0, y1, lo, hi};
     row_block2(a, x0, x1, lds, shT, epi);
-    block_sums<6>(acc, sh);
+    block_sums<6>(epi.acc, sh);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 6; ++q) part[q * a.nblk + blockIdx.x] = acc[q];
+        for (int q = 0; q < 6; ++q) part[q * a.nblk + blockIdx.x] = epi.acc[q];
 }
 
 // Termination column pass for two candidates (termination.cpp:140-209):
 // c^T x, the reduced-cost infimum over the box and dual drift^2.
+struct TermColEpi {
+    struct Pre {
+        double c, lo, hi, x0, x1;
+    };
+    const double* __restrict__ x0;
+    const double* __restrict__ x1;
+    const double* __restrict__ c;
+    const double* __restrict__ lo;
+    const double* __restrict__ hi;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    __device__ Pre load(int j) const { return Pre{c[j], lo[j], hi[j], x0[j], x1[j]}; }
+    __device__ static void one(double s, double cj, double xv, double l, double u, double* o) {
+        const double rc = cj + s;
+        o[0] += cj * xv;
+        double q = rc;
+        if (!isfinite(l)) q = smin(q, 0.0);
+        if (!isfinite(u)) q = smax(q, 0.0);
+        const double dr = rc - q;
+        o[2] += dr * dr;
+        if (q > 0.0) o[1] += q * l;
+        else if (q < 0.0) o[1] += q * u;
+    }
+    __device__ void apply(int, double s0, double s1, const Pre& p) {
+        one(s0, p.c, p.x0, p.lo, p.hi, acc);
+        one(s1, p.c, p.x1, p.lo, p.hi, acc + 3);
+    }
+};
+
 __global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const int32_t* __restrict__ mode,
                                                         const double* __restrict__ y0,
                                                         const double* __restrict__ y1,
@@ -488,27 +576,11 @@ __global__ __launch_bounds__(kThreads) void k_term_cols(Orient<double> at, const
     __shared__ double lds[2 * kTileNnz];
     __shared__ double shT[kWaves];
     __shared__ double sh[6 * kWaves];
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    auto one = [&](int j, double s, double xv, double* o) {
-        const double rc = c[j] + s;
-        const double l = lo[j], u = hi[j];
-        o[0] += c[j] * xv;
-        double q = rc;
-        if (!isfinite(l)) q = smin(q, 0.0);
-        if (!isfinite(u)) q = smax(q, 0.0);
-        const double dr = rc - q;
-        o[2] += dr * dr;
-        if (q > 0.0) o[1] += q * l;
-        else if (q < 0.0) o[1] += q * u;
-    };
-    auto epi = [&](int j, double s0, double s1) {
-        one(j, s0, x0[j], acc);
-        one(j, s1, x1[j], acc + 3);
-    };
+    TermColEpi epi{x0, x1, c, lo, hi};
     row_block2(at, y0, y1, lds, shT, epi);
-    block_sums<6>(acc, sh);
+    block_sums<6>(epi.acc, sh);
     if (threadIdx.x == 0)
-        for (int q = 0; q < 6; ++q) part[q * at.nblk + blockIdx.x] = acc[q];
+        for (int q = 0; q < 6; ++q) part[q * at.nblk + blockIdx.x] = epi.acc[q];
 }
 
 // Segmented fixed-order reduction: for quantity q and instance k,
@@ -598,6 +670,25 @@ __global__ __launch_bounds__(kThreads) void k_farkas_rows(const Seg* __restrict_
 
 // Dual ray, column terms (infeasibility.cpp:62-90): A^T u, its cone drift and
 // the box infimum.
+struct FarkasColEpi {
+    struct Pre {
+        double lo, hi;
+    };
+    const double* __restrict__ lo;
+    const double* __restrict__ hi;
+    double v[2] = {0.0, 0.0};   // violation^2, infimum
+    __device__ Pre load(int j) const { return Pre{lo[j], hi[j]}; }
+    __device__ void apply(int, double rc, const Pre& p) {
+        double q = rc;
+        if (!isfinite(p.lo)) q = smin(q, 0.0);
+        if (!isfinite(p.hi)) q = smax(q, 0.0);
+        const double dr = rc - q;
+        v[0] += dr * dr;
+        if (q > 0.0) v[1] += q * p.lo;
+        else if (q < 0.0) v[1] += q * p.hi;
+    }
+};
+
 __global__ __launch_bounds__(kThreads) void k_farkas_cols(Orient<double> at, const int32_t* __restrict__ mode,
                                                           const double* __restrict__ u,
                                                           const double* __restrict__ lo,
@@ -607,21 +698,12 @@ __global__ __launch_bounds__(kThreads) void k_farkas_cols(Orient<double> at, con
     __shared__ double lds[kTileNnz];
     __shared__ double shT[kWaves];
     __shared__ double sh[2 * kWaves];
-    double v[2] = {0.0, 0.0};   // violation^2, infimum
-    auto epi = [&](int j, double rc) {
-        double q = rc;
-        if (!isfinite(lo[j])) q = smin(q, 0.0);
-        if (!isfinite(hi[j])) q = smax(q, 0.0);
-        const double dr = rc - q;
-        v[0] += dr * dr;
-        if (q > 0.0) v[1] += q * lo[j];
-        else if (q < 0.0) v[1] += q * hi[j];
-    };
+    FarkasColEpi epi{lo, hi};
     row_block(at, u, lds, shT, epi);
-    block_sums<2>(v, sh);
+    block_sums<2>(epi.v, sh);
     if (threadIdx.x == 0) {
-        part[blockIdx.x] = v[0];
-        part[at.nblk + blockIdx.x] = v[1];
+        part[blockIdx.x] = epi.v[0];
+        part[at.nblk + blockIdx.x] = epi.v[1];
     }
 }
 
@@ -660,6 +742,23 @@ __global__ __launch_bounds__(kThreads) void k_ray_cols(const Seg* __restrict__ s
 
 // Primal ray, row terms (infeasibility.cpp:163-178): A d inside the row
 // bounds' recession cone.
+struct RayRowEpi {
+    struct Pre {
+        double lo, hi;
+    };
+    const double* __restrict__ lo;
+    const double* __restrict__ hi;
+    double v[1] = {0.0};
+    __device__ Pre load(int i) const { return Pre{lo[i], hi[i]}; }
+    __device__ void apply(int, double act, const Pre& p) {
+        double q = act;
+        if (isfinite(p.hi)) q = smin(q, 0.0);
+        if (isfinite(p.lo)) q = smax(q, 0.0);
+        const double dr = act - q;
+        v[0] += dr * dr;
+    }
+};
+
 __global__ __launch_bounds__(kThreads) void k_ray_rows(Orient<double> a, const int32_t* __restrict__ mode,
                                                        const double* __restrict__ proj,
                                                        const double* __restrict__ lo,
@@ -668,17 +767,10 @@ __global__ __launch_bounds__(kThreads) void k_ray_rows(Orient<double> a, const i
     __shared__ double lds[kTileNnz];
     __shared__ double shT[kWaves];
     __shared__ double sh[kWaves];
-    double v[1] = {0.0};
-    auto epi = [&](int i, double act) {
-        double q = act;
-        if (isfinite(hi[i])) q = smin(q, 0.0);
-        if (isfinite(lo[i])) q = smax(q, 0.0);
-        const double dr = act - q;
-        v[0] += dr * dr;
-    };
+    RayRowEpi epi{lo, hi};
     row_block(a, proj, lds, shT, epi);
-    block_sums<1>(v, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+    block_sums<1>(epi.v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = epi.v[0];
 }
 
 // ---------------------------------------------------------------------------

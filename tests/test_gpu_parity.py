@@ -163,3 +163,19 @@ def test_invalid_problem_and_options():
         e.load(lp2)
         r = e.solve(_opts(1e-6, termination_check_frequency=0))
     assert r.status == "invalid_problem" and r.status_message == "Invalid PDLP options"
+
+
+def test_graph_replay_equals_eager_launch():
+    """The captured hipGraph replays the same kernels in the same order as the
+    eager launch path: every replay (many per solve, with host work between
+    them) must give bitwise the result of the eager solve."""
+    lp = box_lp(1500, 3750, 5, seed=77)
+    with Engine() as e:
+        e.load(lp)
+        g1 = e.solve(_opts(1e-6))
+        ea = e.solve(_opts(1e-6, eager_launch=1))
+        g2 = e.solve(_opts(1e-6))
+    for r in (ea, g2):
+        assert (r.status, r.iterations, r.step_trials, r.restart_count) == \
+            (g1.status, g1.iterations, g1.step_trials, g1.restart_count)
+        assert np.array_equal(r.primal, g1.primal) and np.array_equal(r.row_dual, g1.row_dual)
