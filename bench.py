@@ -78,12 +78,104 @@ def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads
             "ms_per_iteration": round(per_iter * 1e3, 4), "setup_s": round(t0, 4)}
 
 
+def bench_batch(args) -> int:
+    """Batched workload: every instance solved to 1e-6 in one batched device
+    run (FP32 SpMV and iterates, FP64 KKT checks and hand-off)."""
+    import numpy as np
+    from optimisation_solver_amd import Engine, batch_family, default_options
+    from optimisation_solver_amd import _native
+
+    t = time.perf_counter()
+    lps = batch_family(args.batch_count)
+    Z = sum(lp.nnz for lp in lps)
+    log(f"[bench] generated {len(lps)} LPs (sum m={sum(lp.m for lp in lps)}, sum nnz={Z}) in "
+        f"{time.perf_counter() - t:.1f}s")
+    tol = 1e-6
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=_native.FP32,
+                        iteration_limit=args.iteration_limit, eager_launch=int(args.eager))
+    eng = Engine(0)
+    eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... {run} instances running, iteration {it}"), 20.0)
+    eng.load_batch(lps, _native.FP32)
+    for w in range(args.warmup):
+        eng.solve_batch(o)
+    times, res = [], None
+    for k in range(args.steps):
+        t = time.perf_counter()
+        res = eng.solve_batch(o)
+        times.append(time.perf_counter() - t)
+        st = {}
+        for r in res:
+            st[r.status] = st.get(r.status, 0) + 1
+        its = np.array([r.iterations for r in res])
+        log(f"[bench] step {k}: {times[-1]:.3f}s statuses={st} iterations p50={np.median(its):.0f} max={its.max()}")
+    sec = sum(times) / len(times)
+    per = np.array([r.solve_time_seconds for r in res])
+    ms_p, ms_d = eng.time_trial(_native.FP32, args.kernel_reps)
+    m_tot, n_tot = sum(lp.m for lp in lps), sum(lp.n for lp in lps)
+
+    class _Sz:
+        nnz, m, n = Z, m_tot, n_tot
+
+    bp, bd = trial_bytes(_Sz, 4, 4, 4)
+    gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
+    dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
+    roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom[0],
+            "bytes_per_launch": int(dom[2]), "ms_per_launch": round(dom[1], 5)}
+    eng.close()
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as ol
+    worst = 0.0
+    for i in range(0, len(lps), max(1, len(lps) // 16)):
+        ev = ol.evaluate(lps[i], o, res[i].primal, res[i].row_dual)
+        worst = max(worst, ev["kkt"])
+    cpu = None
+    if not args.no_cpu:
+        # bounded sample: 8 instances spread over the size range, solved to 1e-6
+        order = np.argsort([lp.m for lp in lps])
+        pick = [int(order[q]) for q in np.linspace(0, len(lps) - 1, 8).round().astype(int)]
+        kind = "reference" if ol.have_ref() else "port"
+        threads = min(16, os.cpu_count() or 1) if kind == "reference" else 1
+        oc = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, thread_count=threads)
+        tsum, done = 0.0, []
+        for i in pick:
+            t = time.perf_counter()
+            r = ol.solve(lps[i], oc, "ref" if kind == "reference" else "oracle")
+            tsum += time.perf_counter() - t
+            done.append((lps[i].m, r.status_name, r.iterations))
+        est = tsum / len(pick) * len(lps)
+        cpu = {"value": round(est, 3), "unit": "s", "cores": threads, "kind": kind,
+               "sample": (f"8 of the {len(lps)} instances spread over the size range (m={[d[0] for d in done]}), "
+                          f"each solved to 1e-6 by the {'reference PDLP (oracle/_ref)' if kind == 'reference' else 'oracle'}"
+                          f" in {tsum:.2f}s total; mean per-instance time x {len(lps)}"),
+               "statuses": [d[1] for d in done]}
+        log(f"[bench] cpu: {cpu}")
+    line = {
+        "metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"batch_{len(lps)}_box_lps_m500-20000", "instances": len(lps), "sum_m": m_tot,
+                   "sum_n": n_tot, "sum_nnz": Z, "tolerance": tol,
+                   "optimal": int(sum(r.status == "optimal" for r in res)),
+                   "solves_per_s": round(len(lps) / sec, 2), "p50_instance_s": round(float(np.median(per)), 4),
+                   "p99_instance_s": round(float(np.percentile(per, 99)), 4),
+                   "iterations_p50": int(np.median([r.iterations for r in res])),
+                   "iterations_max": int(max(r.iterations for r in res))},
+        "roofline": roof, "cpu_baseline": cpu, "parity": {"worst_oracle_kkt_of_16": worst},
+    }
+    if cpu:
+        line["speedup_vs_cpu"] = round(cpu["value"] / sec, 2)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3])
+    ap.add_argument("--batch-count", type=int, default=256, help="instances of the batched workload (--config 3)")
     ap.add_argument("--cpu-sample-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=20)
@@ -112,6 +204,8 @@ def main() -> int:
     from optimisation_solver_amd import CONFIGS, Engine, default_options
     from optimisation_solver_amd import _native
 
+    if args.config == 3:
+        return bench_batch(args)
     cfg = CONFIGS[args.config]
     t = time.perf_counter()
     lp = cfg["make"]()
@@ -123,6 +217,7 @@ def main() -> int:
                         iteration_limit=args.iteration_limit, eager_launch=int(args.eager))
 
     eng = Engine(0)
+    eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... iteration {it}, best KKT {kkt:.3e}"), 20.0)
     eng.load(lp, prec)
     for w in range(args.warmup):
         r = eng.solve(o)

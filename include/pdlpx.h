@@ -228,6 +228,15 @@ int pdlpx_time_trial(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
                      double* ms_primal, double* ms_dual);
 
 /*
+ * Progress reporting: during pdlpx_solve / pdlpx_solve_loaded the engine calls
+ * `fn(user, running_instances, max_iteration, best_kkt_of_instance_0)` from
+ * the calling thread at a termination check, at most once every
+ * `min_interval_seconds`.  NULL disables it.  No reference counterpart.
+ */
+typedef void (*pdlpx_progress_fn)(void* user, int32_t running, int64_t iteration, double best_kkt);
+int pdlpx_set_progress(pdlpx_ctx* ctx, pdlpx_progress_fn fn, void* user, double min_interval_seconds);
+
+/*
  * pdlpx_load_batch followed by pdlpx_solve_loaded without warm starts.
  * Per-instance outputs: summaries[i]; x_out[i] (cols_i) and y_out[i] (rows_i)
  * may be NULL.  Options apply to every instance.

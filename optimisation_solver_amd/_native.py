@@ -75,6 +75,8 @@ class Summary(C.Structure):
         return out
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int64, C.c_double)
+
 _ENGINE = None
 _LPGEN = None
 
@@ -102,6 +104,7 @@ def engine() -> C.CDLL:
         lib.pdlpx_scaling.argtypes = [vp, i32, i32, vp, vp, vp, vp]
         lib.pdlpx_load_batch.argtypes = [vp, i32, C.POINTER(Problem), i32]
         lib.pdlpx_solve_loaded.argtypes = [vp, C.POINTER(Options), vp, vp, C.POINTER(Summary), vp, vp]
+        lib.pdlpx_set_progress.argtypes = [vp, PROGRESS_FN, vp, f64]
         lib.pdlpx_time_trial.argtypes = [vp, i32, i32, pf64, pf64]
         lib.pdlpx_solve_batch.argtypes = [vp, i32, C.POINTER(Problem), C.POINTER(Options),
                                           C.POINTER(Summary), vp, vp]

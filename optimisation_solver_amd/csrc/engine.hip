@@ -231,6 +231,9 @@ struct pdlpx_ctx {
     DBuf<DevState> st;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool have32 = false;
+    pdlpx_progress_fn progress = nullptr;
+    void* progressUser = nullptr;
+    double progressInterval = 10.0;
 
     struct GraphEntry {
         hipGraphExec_t exec = nullptr;
@@ -342,6 +345,7 @@ private:
     std::vector<std::vector<double>> rays;   // per instance, filled when a ray is found
     std::vector<std::vector<int32_t>> fl;    // host copies of the flag arrays
     double loopSeconds = 0.0, setupSeconds = 0.0;
+    double lastProgress = 0.0;
 
     int32_t* flagDev(Flag f) const { return X.flags.p + size_t(f) * B; }
     void pushFlags(std::initializer_list<Flag> which) {
@@ -961,6 +965,17 @@ void Run::mainLoop() {
 // Termination check, ray tests, FP32 hand-off and restart for the instances in
 // `atCheck` (pdlp_solver.cpp:354-476).
 void Run::checkMain(const std::vector<int>& atCheck) {
+    if (X.progress && since() - lastProgress >= X.progressInterval) {
+        lastProgress = since();
+        int running = 0;
+        int64_t most = 0;
+        for (int k = 0; k < B; ++k)
+            if (!I[k].mainDone) {
+                ++running;
+                most = std::max<int64_t>(most, hs[k].iteration);
+            }
+        X.progress(X.progressUser, running, most, I[0].best.kkt);
+    }
     std::fill(fl[F_MODE].begin(), fl[F_MODE].end(), 0);
     for (int k : atCheck) fl[F_MODE][k] = (o.use_averaging && hs[k].avg_total != 0.0) ? 2 : 1;
     std::vector<Metrics> cur, avg;
@@ -1706,6 +1721,14 @@ int pdlpx_scaling(pdlpx_ctx* X, int32_t ruiz_iterations, int32_t use_pc, double*
         fetch(primal_scale, X->w64.tau.p, n, X->xNew);
         fetch(dual_scale, X->w64.sig.p, m, X->yNew);
     });
+}
+
+int pdlpx_set_progress(pdlpx_ctx* X, pdlpx_progress_fn fn, void* user, double min_interval_seconds) {
+    if (!X) return PDLPX_E_INVALID;
+    X->progress = fn;
+    X->progressUser = user;
+    X->progressInterval = min_interval_seconds;
+    return PDLPX_OK;
 }
 
 int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_primal, double* ms_dual) {

@@ -164,6 +164,13 @@ def batch_sizes(count: int = 256, seed: int = BASE_SEED + 3000) -> list[tuple[in
     return [(int(mm), int(round(2.5 * mm))) for mm in ms]
 
 
+def batch_family(count: int = 256, seed: int = BASE_SEED + 3000) -> list[Lp]:
+    """The batched workload: `count` box-family LPs with m log-uniform in
+    [500, 20000], n = 2.5 m, 5 nonzeros per column; instance i uses seed
+    seed + 1 + i."""
+    return [box_lp(m, n, 5, seed + 1 + i) for i, (m, n) in enumerate(batch_sizes(count, seed))]
+
+
 # Named workloads (sizes as the benchmark configurations give them).
 CONFIGS = {
     0: dict(name="c0_box_2000x5000", make=lambda: box_lp(2000, 5000, 5, BASE_SEED + 0), tol=1e-4),

@@ -149,6 +149,17 @@ class Engine:
                     "pdlpx_time_trial")
         return a.value, b.value
 
+    def set_progress(self, fn, min_interval: float = 10.0) -> None:
+        """fn(running_instances, iteration, best_kkt) is called from the solve at
+        termination checks, at most every `min_interval` seconds."""
+        if fn is None:
+            self._progress = None
+            self._check(self.lib.pdlpx_set_progress(self.ctx, _native.PROGRESS_FN(), None, min_interval),
+                        "pdlpx_set_progress")
+            return
+        self._progress = _native.PROGRESS_FN(lambda _u, run, it, kkt: fn(run, it, kkt))
+        self._check(self.lib.pdlpx_set_progress(self.ctx, self._progress, None, min_interval), "pdlpx_set_progress")
+
     def close(self) -> None:
         if self.ctx:
             self.lib.pdlpx_destroy(self.ctx)

@@ -72,3 +72,22 @@ def test_batch_with_invalid_member_and_warm_starts():
         warm = e.solve_batch(o, warm_x=[res[0].primal, None, None], warm_y=[res[0].row_dual, None, None])
     assert warm[0].status == "optimal" and warm[0].iterations <= res[0].iterations
     assert warm[2].iterations == res[2].iterations
+
+
+def test_batch_fp32_mixed_precision_reaches_fp64_tolerance():
+    """FP32 SpMV/iterates with FP64 checks and an FP64 hand-off: every
+    instance still reaches the FP64 tolerance as scored by the oracle."""
+    lps = [box_lp(400 + 150 * i, int(2.5 * (400 + 150 * i)), 5, seed=3100 + i) for i in range(5)]
+    o = _opts(1e-6, precision=1)
+    with Engine() as e:
+        e.load_batch(lps, precision=1)
+        res = e.solve_batch(o)
+    for lp, r in zip(lps, res):
+        cpu = ol.solve(lp, _opts(1e-6))
+        print(f"  fp32 batch: {r.status} its={r.iterations} fp32_its={r.diagnostics['fp32_iterations']} "
+              f"obj={r.primal_objective:.10g} cpu obj={cpu.primal_objective:.10g}", flush=True)
+        assert r.status == "optimal"
+        assert r.diagnostics["fp32_iterations"] > 0
+        assert abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-6
+        m = ol.evaluate(lp, o, r.primal, r.row_dual)
+        assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
