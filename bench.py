@@ -4,7 +4,8 @@
 A step is one complete solve (set-up, PDHG iterations, checks) of the seeded
 LP that is already resident in HBM (pdlpx_load runs before the timed region).
 Default workload: the power-law LP (m=200,000, n=500,000, ~5.3M nnz, FP64,
-1e-6); --config 2 selects the 1M x 2M FP32 mixed-precision LP, --config 0 the
+1e-6); --config 2 selects the 1M x 2M LP (FP32 matrix values, FP64 iterates),
+--config 3 the batch of 256 LPs, --config 0 the
 CPU-runnable 2,000 x 5,000 LP.
 
 Printed (stdout, one JSON line): the contract fields plus
@@ -30,6 +31,10 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+PRECISIONS = {"fp64": 0, "fp32": 1, "mixed": 2}   # enum pdlpx_precision
+# (matrix value, iterate vector, data vector) bytes of each precision
+SIZES = {"fp64": (8, 8, 8), "fp32": (4, 4, 4), "mixed": (4, 8, 8)}
+DTYPES = {"fp64": "f64", "fp32": "f32 (f64 checks)", "mixed": "f32 values, f64 iterates"}
 METRIC = "time to 1e-6 rel. KKT & SpMV HBM GB/s; FP32+warm-start vs plain FP64 PDLP"
 
 
@@ -80,10 +85,9 @@ def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads
 
 def bench_batch(args) -> int:
     """Batched workload: every instance solved to 1e-6 in one batched device
-    run (FP32 SpMV and iterates, FP64 KKT checks and hand-off)."""
+    run (default: FP32 matrix values with FP64 iterates and checks)."""
     import numpy as np
     from optimisation_solver_amd import Engine, batch_family, default_options
-    from optimisation_solver_amd import _native
 
     t = time.perf_counter()
     lps = batch_family(args.batch_count)
@@ -91,11 +95,13 @@ def bench_batch(args) -> int:
     log(f"[bench] generated {len(lps)} LPs (sum m={sum(lp.m for lp in lps)}, sum nnz={Z}) in "
         f"{time.perf_counter() - t:.1f}s")
     tol = 1e-6
-    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=_native.FP32,
+    pname = args.precision or "mixed"
+    prec = PRECISIONS[pname]
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=prec,
                         iteration_limit=args.iteration_limit, eager_launch=int(args.eager))
     eng = Engine(0)
     eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... {run} instances running, iteration {it}"), 20.0)
-    eng.load_batch(lps, _native.FP32)
+    eng.load_batch(lps, prec)
     for w in range(args.warmup):
         eng.solve_batch(o)
     times, res = [], None
@@ -110,13 +116,13 @@ def bench_batch(args) -> int:
         log(f"[bench] step {k}: {times[-1]:.3f}s statuses={st} iterations p50={np.median(its):.0f} max={its.max()}")
     sec = sum(times) / len(times)
     per = np.array([r.solve_time_seconds for r in res])
-    ms_p, ms_d = eng.time_trial(_native.FP32, args.kernel_reps)
+    ms_p, ms_d = eng.time_trial(prec, args.kernel_reps)
     m_tot, n_tot = sum(lp.m for lp in lps), sum(lp.n for lp in lps)
 
     class _Sz:
         nnz, m, n = Z, m_tot, n_tot
 
-    bp, bd = trial_bytes(_Sz, 4, 4, 4)
+    bp, bd = trial_bytes(_Sz, *SIZES[pname])
     gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
     dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
     roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -153,8 +159,9 @@ def bench_batch(args) -> int:
     line = {
         "metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-        "config": {"workload": f"batch_{len(lps)}_box_lps_m500-20000", "instances": len(lps), "sum_m": m_tot,
+        "vs_baseline": None, "dtype": DTYPES[pname], "data": "synthetic",
+        "config": {"workload": f"batch_{len(lps)}_box_lps_m500-20000", "precision": pname,
+                   "instances": len(lps), "sum_m": m_tot,
                    "sum_n": n_tot, "sum_nnz": Z, "tolerance": tol,
                    "optimal": int(sum(r.status == "optimal" for r in res)),
                    "solves_per_s": round(len(lps) / sec, 2), "p50_instance_s": round(float(np.median(per)), 4),
@@ -178,6 +185,9 @@ def main() -> int:
     ap.add_argument("--batch-count", type=int, default=256, help="instances of the batched workload (--config 3)")
     ap.add_argument("--cpu-sample-iters", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default=None,
+                    help="PDHG arithmetic: fp64 (reference), fp32 (FP32 values and iterates), mixed (FP32 "
+                         "matrix values, FP64 iterates); default mixed for --config 2 and 3, fp64 otherwise")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--eager", action="store_true",
                     help="launch trial kernels eagerly instead of through the captured hipGraph")
@@ -202,7 +212,6 @@ def main() -> int:
         return 2
 
     from optimisation_solver_amd import CONFIGS, Engine, default_options
-    from optimisation_solver_amd import _native
 
     if args.config == 3:
         return bench_batch(args)
@@ -211,8 +220,8 @@ def main() -> int:
     lp = cfg["make"]()
     log(f"[bench] generated {cfg['name']}: m={lp.m} n={lp.n} nnz={lp.nnz} in {time.perf_counter() - t:.1f}s")
     tol = cfg["tol"]
-    fp32 = args.config == 2
-    prec = _native.FP32 if fp32 else _native.FP64
+    pname = args.precision or ("mixed" if args.config == 2 else "fp64")
+    prec = PRECISIONS[pname]
     o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=prec,
                         iteration_limit=args.iteration_limit, eager_launch=int(args.eager))
 
@@ -232,11 +241,10 @@ def main() -> int:
             f"{res.dual_residual:.2e},{res.relative_gap:.2e}) fp32_its={res.diagnostics['fp32_iterations']}")
     sec = sum(times) / len(times)
 
-    # dominant kernel: per-launch time with hipEvents on the engine's stream
-    phase_prec = _native.FP64 if (not fp32 or res.diagnostics["fp32_iterations"] < res.iterations) else _native.FP32
-    ms_p, ms_d = eng.time_trial(_native.FP32 if fp32 else _native.FP64, args.kernel_reps)
-    sv = 4 if fp32 else 8
-    bp, bd = trial_bytes(lp, sv, sv, sv)
+    # dominant kernel: per-launch time with hipEvents on the engine's stream,
+    # in the precision the bulk of the iterations ran in
+    ms_p, ms_d = eng.time_trial(prec, args.kernel_reps)
+    bp, bd = trial_bytes(lp, *SIZES[pname])
     gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
     dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
     traffic = None
@@ -249,7 +257,6 @@ def main() -> int:
             "other_kernel": {"name": "pdhg_dual_kernel" if dom[0] == "pdhg_primal_kernel" else "pdhg_primal_kernel",
                              "ms_per_launch": round(ms_d if dom[0] == "pdhg_primal_kernel" else ms_p, 5),
                              "achieved": round(gbs_d if dom[0] == "pdhg_primal_kernel" else gbs_p, 1)}}
-    del phase_prec
 
     # parity: re-score the GPU point with the oracle's KKT evaluator
     sys.path.insert(0, str(ROOT / "tests"))
@@ -270,10 +277,11 @@ def main() -> int:
     line = {
         "metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if fp32 else "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[pname], "data": "synthetic",
         "config": {"workload": cfg["name"], "m": lp.m, "n": lp.n, "nnz": lp.nnz, "tolerance": tol,
                    "status": res.status, "iterations": res.iterations, "step_trials": res.step_trials,
-                   "restarts": res.restart_count, "fp32_iterations": res.diagnostics["fp32_iterations"],
+                   "precision": pname, "restarts": res.restart_count,
+                   "fp32_iterations": res.diagnostics["fp32_iterations"],
                    "device_loop_s": round(res.diagnostics["device_loop_seconds"], 4),
                    "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,

@@ -9,7 +9,8 @@ namespace pdlp {
 
 enum class Precision {
     Fp64,   // matrix, iterates and products in FP64
-    Fp32    // FP32 matrix/iterates/products, FP64 checks and a final FP64 phase
+    Fp32,   // FP32 matrix/iterates/products, FP64 checks and a final FP64 phase
+    Mixed   // FP32 matrix values, FP64 iterates and sums, final FP64-value phase
 };
 
 struct PdlpOptions {
@@ -61,8 +62,8 @@ struct PdlpOptions {
     Precision precision = Precision::Fp64;
     // The FP32 phase hands over to FP64 once the best KKT score is within
     // this factor of the tolerance, or after this many checks without progress.
-    double fp32HandoffFactor = 10.0;
-    int fp32StallChecks = 20;
+    double fp32HandoffFactor = 1.0;
+    int fp32StallChecks = 50;
     // HIP device ordinal.
     int device = 0;
 };

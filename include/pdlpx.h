@@ -51,8 +51,11 @@ enum pdlpx_status {
 /* Arithmetic of the PDHG iterations. */
 enum pdlpx_precision {
     PDLPX_FP64 = 0,   /* matrix, iterates and SpMV in FP64 (reference arithmetic) */
-    PDLPX_FP32 = 1    /* FP32 matrix/iterates/SpMV; FP64 KKT checks, gap sums,
+    PDLPX_FP32 = 1,   /* FP32 matrix/iterates/SpMV; FP64 KKT checks, gap sums,
                          Ruiz and a final FP64 PDHG phase */
+    PDLPX_MIXED = 2   /* FP32 matrix values (half the streamed bytes of the
+                         SpMVs) with FP64 iterates, sums and checks; a final
+                         FP64-value phase if the FP32 operator stalls */
 };
 
 /*
@@ -116,10 +119,10 @@ typedef struct pdlpx_options {
     int32_t polishing_iterations;
     /* ---- engine extensions (no reference counterpart) ---- */
     int32_t precision;                    /* enum pdlpx_precision */
-    double fp32_handoff_factor;           /* FP32 phase hands over to FP64 once the
-                                             best KKT score <= factor * tolerance */
-    int32_t fp32_stall_checks;            /* ... or after this many checks without
-                                             improving the best score */
+    double fp32_handoff_factor;           /* if > 1, the FP32 phase hands over to FP64
+                                             once the best KKT <= factor * tolerance */
+    int32_t fp32_stall_checks;            /* ... and always after max(this, checks/4)
+                                             checks without improving the best score */
     int32_t eager_launch;                 /* 1: launch the trial kernels directly on the
                                              stream instead of replaying a captured
                                              hipGraph (same kernels, same order) */

@@ -20,7 +20,7 @@ pf64 = C.POINTER(C.c_double)
 PDLPX_OK = 0
 STATUS_NAMES = ["optimal", "infeasible", "unbounded", "iteration_limit", "time_limit",
                 "numerical_failure", "invalid_problem"]
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
 
 
 class Problem(C.Structure):

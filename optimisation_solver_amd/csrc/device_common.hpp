@@ -36,12 +36,15 @@ constexpr int kTileRows = kThreads;              // rows per block (one thread e
 constexpr int kReduceThreads = 1024;             // single-workgroup final reductions
 
 // Matrix orientation as the kernels see it: rows of this orientation.
+// Each row block carries its row range and nonzero range in one 16-byte
+// record, so a workgroup issues its index/value loads right after one
+// (cached) metadata load instead of after a dependent load of ptr[].
 template <typename T>
 struct Orient {
     const int32_t* ptr;     // nrows + 1
     const int32_t* idx;     // nnz
     const T* val;           // nnz
-    const int32_t* blk;     // nblk + 1 row-block starts
+    const int4* meta;       // nblk: {first row, end row, first nonzero, end nonzero}
     const int32_t* binst;   // nblk: batch instance owning each block
     int32_t nblk;
     int32_t nrows;
@@ -104,11 +107,11 @@ __device__ inline T block_total(T v, T* sh) {
 
 // Products of one chunk of up to kTileNnz nonzeros [k0, k1) into lds[0..k1-k0):
 // all index/value loads first, then all gathers.
-template <typename T>
-__device__ inline void stage_chunk(const Orient<T>& M, const T* __restrict__ v, int k0, int k1, T* lds) {
+template <typename V, typename T>
+__device__ inline void stage_chunk(const Orient<V>& M, const T* __restrict__ v, int k0, int k1, T* lds) {
     const int t = threadIdx.x;
     int col[kPerThread];
-    T a[kPerThread];
+    V a[kPerThread];
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const int k = k0 + t + q * kThreads;
@@ -126,7 +129,7 @@ __device__ inline void stage_chunk(const Orient<T>& M, const T* __restrict__ v, 
 #pragma unroll
     for (int q = 0; q < kPerThread; ++q) {
         const int k = k0 + t + q * kThreads;
-        if (k < k1) lds[k - k0] = a[q] * g[q];
+        if (k < k1) lds[k - k0] = static_cast<T>(a[q]) * g[q];
     }
 }
 
@@ -187,12 +190,17 @@ __device__ inline T group_sum(T v, int tpr) {
 // `lds` holds kTileNnz elements of T, `shT` kWaves elements of T.  Each row
 // of the block is summed from LDS by a group of threads_per_row(rows) lanes
 // (strided, then a shuffle tree), so a block of few long rows still uses the
-// whole workgroup; the group's first lane runs the epilogue.
-template <typename T, typename Epi>
-__device__ inline void row_block(const Orient<T>& M, const T* __restrict__ v, T* lds, T* shT, Epi& epi) {
-    const int b = blockIdx.x;
-    const int r0 = M.blk[b];
-    const int r1 = M.blk[b + 1];
+// whole workgroup; the group's first lane runs the epilogue.  Matrix values
+// are of type V (stored precision), products and sums of type T.
+// `bm` is this block's metadata record (M.meta[blockIdx.x]), loaded by the
+// caller together with its instance state.
+template <typename V, typename T, typename Epi>
+__device__ inline void row_block(const Orient<V>& M, const int4 bm, const T* __restrict__ v, T* lds, T* shT,
+                                 Epi& epi) {
+    const int r0 = bm.x;
+    const int r1 = bm.y;
+    const int k0 = bm.z;
+    const int k1 = bm.w;
     const int t = threadIdx.x;
     const int tpr = threads_per_row(r1 - r0);
     const int rr = t / tpr;
@@ -206,14 +214,12 @@ __device__ inline void row_block(const Orient<T>& M, const T* __restrict__ v, T*
         a = M.ptr[r];
         e = M.ptr[r + 1];
     }
-    const int k0 = M.ptr[r0];
-    const int k1 = M.ptr[r1];
     if (k1 - k0 > kTileNnz) {   // one long row: stream it through the workgroup
         T s = T(0);
         for (int c0 = k0; c0 < k1; c0 += kTileNnz) {
             const int c1 = c0 + kTileNnz < k1 ? c0 + kTileNnz : k1;
             int col[kPerThread];
-            T av[kPerThread];
+            V av[kPerThread];
 #pragma unroll
             for (int q = 0; q < kPerThread; ++q) {
                 const int k = c0 + t + q * kThreads;
@@ -225,7 +231,7 @@ __device__ inline void row_block(const Orient<T>& M, const T* __restrict__ v, T*
 #pragma unroll
             for (int q = 0; q < kPerThread; ++q) {
                 const int k = c0 + t + q * kThreads;
-                if (k < c1) s += av[q] * v[col[q]];
+                if (k < c1) s += static_cast<T>(av[q]) * v[col[q]];
             }
         }
         const T total = block_total(s, shT);
@@ -247,9 +253,11 @@ __device__ inline void row_block(const Orient<T>& M, const T* __restrict__ v, T*
 template <typename T, typename Epi>
 __device__ inline void row_block2(const Orient<T>& M, const T* __restrict__ v0, const T* __restrict__ v1, T* lds,
                                   T* shT, Epi& epi) {
-    const int b = blockIdx.x;
-    const int r0 = M.blk[b];
-    const int r1 = M.blk[b + 1];
+    const int4 bm = M.meta[blockIdx.x];
+    const int r0 = bm.x;
+    const int r1 = bm.y;
+    const int k0 = bm.z;
+    const int k1 = bm.w;
     const int t = threadIdx.x;
     const int tpr = threads_per_row(r1 - r0);
     const int rr = t / tpr;
@@ -263,8 +271,6 @@ __device__ inline void row_block2(const Orient<T>& M, const T* __restrict__ v0, 
         a = M.ptr[r];
         e = M.ptr[r + 1];
     }
-    const int k0 = M.ptr[r0];
-    const int k1 = M.ptr[r1];
     if (k1 - k0 > kTileNnz) {
         T s0 = T(0), s1 = T(0);
         for (int k = k0 + t; k < k1; k += kThreads) {

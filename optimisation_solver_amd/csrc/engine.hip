@@ -133,12 +133,13 @@ void makeSegments(const std::vector<int32_t>& instStart, std::vector<Seg>& segs,
 }
 
 struct OrientBuf {
-    DBuf<int32_t> ptr, idx, blk, binst, ibeg;
+    DBuf<int32_t> ptr, idx, binst, ibeg;
+    DBuf<int4> meta;    // per block: first row, end row, first nonzero, end nonzero
     DBuf<double> val;   // original values
     int32_t nblk = 0, nrows = 0;
     template <typename T>
     Orient<T> view(const T* values) const {
-        return Orient<T>{ptr.p, idx.p, values, blk.p, binst.p, nblk, nrows};
+        return Orient<T>{ptr.p, idx.p, values, meta.p, binst.p, nblk, nrows};
     }
 };
 
@@ -400,29 +401,41 @@ private:
     std::vector<double> powerIteration(std::vector<char>& converged);
 
     // ---- trial slots ----
-    template <typename T>
-    void launchPrimal() {
-        const Working<T>& w = work<T>();
-        k_primal<T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<T>(w.vc.p), w.cols(), iters<T>().view(), X.st.p,
-                                                   X.part_p.p);
+    // Phase bit p of `phases` (1 << DevState::phase): 0 FP64, 1 FP32, 2 FP32
+    // values with FP64 iterates.
+    template <typename V>
+    const V* colValues() const {
+        if constexpr (sizeof(V) == 4) return X.w32.vc.p;
+        else return X.w64.vc.p;
     }
-    template <typename T>
+    template <typename V>
+    const V* rowValues() const {
+        if constexpr (sizeof(V) == 4) return X.w32.vr.p;
+        else return X.w64.vr.p;
+    }
+    template <typename V, typename T>
+    void launchPrimal() {
+        k_primal<V, T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<V>(colValues<V>()), work<T>().cols(),
+                                                      iters<T>().view(), X.st.p, X.part_p.p);
+    }
+    template <typename V, typename T>
     void launchDual() {
-        const Working<T>& w = work<T>();
-        k_dual<T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<T>(w.vr.p), w.rows(), iters<T>().view(), X.st.p,
-                                                X.part_d.p, X.part_i.p);
+        k_dual<V, T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<V>(rowValues<V>()), work<T>().rows(), iters<T>().view(),
+                                                   X.st.p, X.part_d.p, X.part_i.p);
     }
     template <typename T>
     void launchAverage() {
         k_average<T><<<nsegs(), kThreads, 0, s>>>(X.st.p, iters<T>().view(), X.segs.p, X.ncs);
     }
     void launchSlot(int phases) {
-        if (phases & 1) launchPrimal<double>();
-        if (phases & 2) launchPrimal<float>();
-        if (phases & 1) launchDual<double>();
-        if (phases & 2) launchDual<float>();
+        if (phases & 1) launchPrimal<double, double>();
+        if (phases & 2) launchPrimal<float, float>();
+        if (phases & 4) launchPrimal<float, double>();
+        if (phases & 1) launchDual<double, double>();
+        if (phases & 2) launchDual<float, float>();
+        if (phases & 4) launchDual<float, double>();
         k_decide<<<B, kReduceThreads, 0, s>>>(X.st.p, X.AT.ibeg.p, X.A.ibeg.p, X.part_p.p, X.part_d.p, X.part_i.p);
-        if (phases & 1) launchAverage<double>();
+        if (phases & 5) launchAverage<double>();
         if (phases & 2) launchAverage<float>();
     }
     hipGraphExec_t graphFor(int phases, int slots) {
@@ -450,7 +463,7 @@ private:
         int64_t left = 0;
         for (int k = 0; k < B; ++k)
             if (!hs[k].done) {
-                phases |= hs[k].phase ? 2 : 1;
+                phases |= 1 << hs[k].phase;
                 left = std::max<int64_t>(left, hs[k].target - hs[k].iteration);
             }
         if (!phases) return;
@@ -508,7 +521,7 @@ private:
         std::fill(fl[F_MODE64].begin(), fl[F_MODE64].end(), 0);
         std::fill(fl[F_MODE32].begin(), fl[F_MODE32].end(), 0);
         for (int k = 0; k < B; ++k)
-            if (fl[F_MODE][k]) (hs[k].phase ? fl[F_MODE32] : fl[F_MODE64])[k] = fl[F_MODE][k];
+            if (fl[F_MODE][k]) (hs[k].phase == 1 ? fl[F_MODE32] : fl[F_MODE64])[k] = fl[F_MODE][k];
         pushFlags({F_MODE, F_MODE64, F_MODE32});
         if (anyOf(fl[F_MODE64]))
             k_to_original<double><<<nsegs(), kThreads, 0, s>>>(X.segs.p, X.ncs, X.cs.p, X.rs.p, X.it64.view(), X.st.p,
@@ -628,9 +641,20 @@ private:
         h.omega = 1.0;
     }
 
+    // A x for the flagged instances whose iterates are of type T, with the
+    // matrix values of each instance's phase.
     template <typename T>
     void refresh(const int32_t* flag) {
-        k_refresh<T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<T>(work<T>().vr.p), iters<T>().view(), X.st.p, flag);
+        if constexpr (sizeof(T) == 4) {
+            k_refresh<float, float><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<float>(X.w32.vr.p), X.it32.view(), X.st.p,
+                                                                  flag);
+        } else {
+            k_refresh<double, double><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.w64.vr.p), X.it64.view(),
+                                                                    X.st.p, flag);
+            if (X.have32)
+                k_refresh<float, double><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<float>(X.w32.vr.p), X.it64.view(),
+                                                                       X.st.p, flag);
+        }
     }
 
     // fn(double{}, flags64) and/or fn(float{}, flags32) for the precisions
@@ -650,7 +674,7 @@ private:
         for (int k = 0; k < B; ++k) {
             a64[k] = a32[k] = 0;
             if (!fl[F_SEL][k]) continue;
-            (hs[k].phase ? a32 : a64)[k] = 1;
+            (hs[k].phase == 1 ? a32 : a64)[k] = 1;
             hs[k].parity = 0;
             hs[k].avg_total = 0.0;
         }
@@ -729,7 +753,7 @@ void Run::setup() {
         }
     }
     const Working<double>& w = X.w64;
-    const bool f32 = o.precision == PDLPX_FP32;
+    const bool f32 = o.precision != PDLPX_FP64;
     float* v32r = f32 ? X.w32.vr.p : nullptr;
     float* v32c = f32 ? X.w32.vc.p : nullptr;
     k_scale_values<<<gridFor(m, bt), bt, 0, s>>>(m, X.A.ptr.p, X.A.idx.p, X.A.val.p, X.rs.p, X.cs.p, w.vr.p, v32r);
@@ -888,7 +912,10 @@ void Run::initStart(const double* const* warmX, const double* const* warmY) {
         }
     }
     // the running instances' phase, start iterate, A x and anchors
-    if (o.precision == PDLPX_FP32) {
+    if (o.precision == PDLPX_MIXED) {
+        for (int k = 0; k < B; ++k)
+            if (!I[k].finished) hs[k].phase = 2;
+    } else if (o.precision == PDLPX_FP32) {
         const IterBuf<float>& F = X.it32;
         const int g = std::min(2048, gridFor(std::max(n, m)));
         k_convert<float, double><<<g, kThreads, 0, s>>>(n, D.x[0].p, F.x[0].p);
@@ -898,7 +925,7 @@ void Run::initStart(const double* const* warmX, const double* const* warmY) {
             if (!I[k].finished) hs[k].phase = 1;
     }
     for (int k = 0; k < B; ++k) {
-        fl[F_AUX64][k] = (!I[k].finished && hs[k].phase == 0) ? 3 : 0;
+        fl[F_AUX64][k] = (!I[k].finished && hs[k].phase != 1) ? 3 : 0;
         fl[F_AUX32][k] = (!I[k].finished && hs[k].phase == 1) ? 3 : 0;
     }
     pushState();
@@ -1021,12 +1048,16 @@ void Run::checkMain(const std::vector<int>& atCheck) {
             I[k].finishTime = since();
             continue;
         }
-        if (hs[k].phase == 1) {   // FP32 -> FP64 hand-off test
+        if (hs[k].phase != 0) {   // FP32 -> FP64 hand-off test
             const double tol = std::max(o.primal_tolerance, std::max(o.dual_tolerance, o.gap_tolerance));
+            // A stall is judged against the solve's own pace: restarts come
+            // every few thousand iterations, so the window grows with the
+            // checks done so far.
             I[k].stall = I[k].best.kkt < I[k].lastBest ? 0 : I[k].stall + 1;
             I[k].lastBest = std::min(I[k].lastBest, I[k].best.kkt);
-            if (I[k].best.kkt <= o.fp32_handoff_factor * tol ||
-                (o.fp32_stall_checks > 0 && I[k].stall >= o.fp32_stall_checks))
+            const int window = std::max(o.fp32_stall_checks, I[k].checks / 4);
+            if ((o.fp32_handoff_factor > 1.0 && I[k].best.kkt <= o.fp32_handoff_factor * tol) ||
+                (o.fp32_stall_checks > 0 && I[k].stall >= window))
                 handoff[k] = 1;
         }
     }
@@ -1061,7 +1092,7 @@ void Run::checkMain(const std::vector<int>& atCheck) {
     }
     if (anyOf(pick)) {
         for (int k = 0; k < B; ++k) {
-            fl[F_AUX64][k] = hs[k].phase == 0 ? pick[k] : 0;
+            fl[F_AUX64][k] = hs[k].phase != 1 ? pick[k] : 0;
             fl[F_AUX32][k] = hs[k].phase == 1 ? pick[k] : 0;
         }
         pushFlags({F_AUX64, F_AUX32});
@@ -1076,7 +1107,7 @@ void Run::checkMain(const std::vector<int>& atCheck) {
                                {X.part_seg3.p, X.colSegBeg.p}, {X.part_seg3.p, X.rowSegBegAll.p}});
         for (int k = 0; k < B; ++k) {
             if (!pick[k]) continue;
-            const int q = hs[k].phase ? 2 : 0;
+            const int q = hs[k].phase == 1 ? 2 : 0;
             reweight(k, std::sqrt(d[q][k]), std::sqrt(d[q + 1][k]));
         }
         // adopt the average where chosen, then anchors = current, average = 0
@@ -1103,19 +1134,36 @@ void Run::checkMain(const std::vector<int>& atCheck) {
             ++I[k].restarts;
         }
     }
-    // ---- hand over to FP64 from the best point, as after a restart ----
-    if (std::any_of(handoff.begin(), handoff.end(), [](char c) { return c != 0; })) {
-        std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
-        for (int k = 0; k < B; ++k) {
-            if (!handoff[k]) continue;
+    // ---- hand over to FP64 ----
+    // FP32 values with FP64 iterates: keep the iterate, switch the matrix to
+    // FP64 and recompute A x.  FP32 iterates: restart from the best point in
+    // FP64, as after a restart.
+    bool anyInPlace = false, anyRestart = false;
+    std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
+    std::fill(fl[F_AUX64].begin(), fl[F_AUX64].end(), 0);
+    for (int k = 0; k < B; ++k) {
+        if (!handoff[k]) continue;
+        if (hs[k].phase == 2) {
+            hs[k].phase = 0;
+            fl[F_AUX64][k] = 1;
+            anyInPlace = true;
+        } else {
             fl[F_SEL][k] = 1;
             hs[k].phase = 0;
             I[k].baseline = I[k].best.kkt;
             I[k].prevScore = kInf;
             I[k].sinceRestart = 0;
+            anyRestart = true;
         }
-        adopt(X.bestx.p, X.besty.p);
     }
+    if (anyInPlace) {
+        pushState();
+        pushFlags({F_AUX64});
+        k_refresh<double, double><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.w64.vr.p), X.it64.view(), X.st.p,
+                                                                flagDev(F_AUX64));
+        HIPCHK(hipGetLastError());
+    }
+    if (anyRestart) adopt(X.bestx.p, X.besty.p);
 }
 
 // FeasibilityPolisher::polish (feasibility_polishing.cpp:13-145) for the
@@ -1311,12 +1359,15 @@ void uploadOrient(OrientBuf& ob, const std::vector<int32_t>& ptr, const std::vec
     partitionRows(ptr, instStart, blk, binst, ibeg);
     ob.nrows = static_cast<int32_t>(ptr.size()) - 1;
     ob.nblk = static_cast<int32_t>(binst.size());
+    std::vector<int4> meta(ob.nblk);
+    for (int b = 0; b < ob.nblk; ++b) meta[b] = make_int4(blk[b], blk[b + 1], ptr[blk[b]], ptr[blk[b + 1]]);
     ob.ptr.upload(ptr.data(), ptr.size(), s);
-    ob.blk.upload(blk.data(), blk.size(), s);
+    ob.meta.upload(meta.data(), meta.size(), s);
     ob.binst.upload(binst.data(), binst.size(), s);
     ob.ibeg.upload(ibeg.data(), ibeg.size(), s);
     ob.idx.upload(idx.data(), idx.size(), s);
     ob.val.upload(val.data(), val.size(), s);
+    HIPCHK(hipStreamSynchronize(s));   // the local staging vectors die here
 }
 
 void allocWork(pdlpx_ctx& X, bool fp32) {
@@ -1479,7 +1530,7 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     X.colSegBeg.upload(cbeg.data(), cbeg.size(), s);
     X.rowSegBeg.upload(rbeg.data(), rbeg.size(), s);
     X.rowSegBegAll.upload(rall.data(), rall.size(), s);
-    allocWork(X, precision == PDLPX_FP32);
+    allocWork(X, precision != PDLPX_FP64);
     HIPCHK(hipStreamSynchronize(s));   // host staging vectors die here
 }
 
@@ -1526,7 +1577,7 @@ void solveLoaded(pdlpx_ctx& X, const pdlpx_options& o, const double* const* wx, 
     for (int u = 0; u < users; ++u)
         if (!X.info[u].valid) rejectAll(sums[u], X.info[u].invalid, secs());
     if (X.B == 0) return;
-    if (o.precision == PDLPX_FP32 && !X.have32) allocWork(X, true);
+    if (o.precision != PDLPX_FP64 && !X.have32) allocWork(X, true);
     Run run(X, o);
     run.go(wx, wy, sums, xOut, yOut, rayOut);
 }
@@ -1569,8 +1620,8 @@ void pdlpx_default_options(pdlpx_options* o) {   // pdlp_options.h:7-75
     o->primal_weight_smoothing = 0.50;
     o->polishing_iterations = 5000;
     o->precision = PDLPX_FP64;
-    o->fp32_handoff_factor = 10.0;
-    o->fp32_stall_checks = 20;
+    o->fp32_handoff_factor = 1.0;
+    o->fp32_stall_checks = 50;
 }
 
 int pdlpx_create(pdlpx_ctx** out, int device) {
@@ -1735,7 +1786,7 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
     if (!ms_primal || !ms_dual || reps <= 0) return PDLPX_E_INVALID;
     return guarded(X, [&] {
         if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
-        if (precision == PDLPX_FP32 && !X->have32) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
+        if (precision != PDLPX_FP64 && !X->have32) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
         hipStream_t s = X->stream;
         const int B = X->B;
         std::vector<DevState> saved(B), probe;
@@ -1744,7 +1795,7 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
         probe = saved;
         for (auto& h : probe) {
             h.done = 0;
-            h.phase = precision == PDLPX_FP32 ? 1 : 0;
+            h.phase = precision;
         }
         HIPCHK(hipMemcpyAsync(X->st.p, probe.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
         auto timeIt = [&](auto&& launch) {
@@ -1757,8 +1808,18 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
             HIPCHK(hipEventElapsedTime(&ms, X->ev0, X->ev1));
             return static_cast<double>(ms) / reps;
         };
-        auto both = [&](auto tag) {
-            using T = decltype(tag);
+        auto both = [&](auto vtag, auto ttag) {
+            using V = decltype(vtag);
+            using T = decltype(ttag);
+            const V* vc;
+            const V* vr;
+            if constexpr (sizeof(V) == 4) {
+                vc = X->w32.vc.p;
+                vr = X->w32.vr.p;
+            } else {
+                vc = X->w64.vc.p;
+                vr = X->w64.vr.p;
+            }
             const Working<T>* wp;
             const IterBuf<T>* ip;
             if constexpr (sizeof(T) == 4) {
@@ -1768,19 +1829,18 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
                 wp = &X->w64;
                 ip = &X->it64;
             }
-            const Working<T>& w = *wp;
-            const IterBuf<T>& it = *ip;
             *ms_primal = timeIt([&] {
-                k_primal<T><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<T>(w.vc.p), w.cols(), it.view(), X->st.p,
-                                                            X->part_p.p);
+                k_primal<V, T><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<V>(vc), wp->cols(), ip->view(), X->st.p,
+                                                               X->part_p.p);
             });
             *ms_dual = timeIt([&] {
-                k_dual<T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<T>(w.vr.p), w.rows(), it.view(), X->st.p,
-                                                         X->part_d.p, X->part_i.p);
+                k_dual<V, T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<V>(vr), wp->rows(), ip->view(), X->st.p,
+                                                            X->part_d.p, X->part_i.p);
             });
         };
-        if (precision == PDLPX_FP32) both(float{});
-        else both(double{});
+        if (precision == PDLPX_FP32) both(float{}, float{});
+        else if (precision == PDLPX_MIXED) both(float{}, double{});
+        else both(double{}, double{});
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(X->st.p, saved.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));

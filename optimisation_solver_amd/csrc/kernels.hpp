@@ -47,7 +47,8 @@ struct DevState {
     int32_t adaptive;
     int32_t averaging;
     int32_t max_trials;
-    int32_t phase;       // 0 = FP64 iterates, 1 = FP32 iterates
+    int32_t phase;       // 0 = FP64 values and iterates, 1 = FP32 values and
+                         // iterates, 2 = FP32 values with FP64 iterates
     int32_t pad;
 };
 
@@ -60,16 +61,27 @@ struct Seg {
     int32_t pad;
 };
 
-template <typename T>
+// Phase of a (matrix value type, iterate type) pair.
+template <typename V, typename T>
 struct PhaseOf;
 template <>
-struct PhaseOf<double> {
+struct PhaseOf<double, double> {
     static constexpr int value = 0;
 };
 template <>
-struct PhaseOf<float> {
+struct PhaseOf<float, float> {
     static constexpr int value = 1;
 };
+template <>
+struct PhaseOf<float, double> {
+    static constexpr int value = 2;
+};
+
+// Whether iterates of type T belong to an instance in `phase`.
+template <typename T>
+__device__ inline bool iterates_of(int phase) {
+    return (phase == 1) == (sizeof(T) == 4);
+}
 
 template <typename T>
 struct ColData {   // per-column data of the working problem
@@ -130,18 +142,19 @@ struct PrimalEpi {
 // Primal half of a trial (pdhg_kernel.cpp:47-104): per column j of the CSC
 // orientation, g_j = c_j + (A^T y)_j and x'_j = clip(x_j - (eta/omega) T_j g_j);
 // accumulates sum dx_j^2 / T_j.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_primal(Orient<T> at, ColData<T> cd, Iterates<T> it,
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_primal(Orient<V> at, ColData<T> cd, Iterates<T> it,
                                                      const DevState* __restrict__ states,
                                                      double* __restrict__ part) {
+    const int4 bm = at.meta[blockIdx.x];
     const DevState& st = states[at.binst[blockIdx.x]];
-    if (st.done || st.phase != PhaseOf<T>::value) return;
+    if (st.done || st.phase != PhaseOf<V, T>::value) return;
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     __shared__ double sh[kWaves];
     const int p = st.parity;
     PrimalEpi<T> epi{cd, it.x[p], it.x[p ^ 1], static_cast<T>(st.eta_trial / st.omega)};
-    row_block(at, it.y[p], lds, shT, epi);
+    row_block(at, bm, it.y[p], lds, shT, epi);
     double v[1] = {epi.acc};
     block_sums<1>(v, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = v[0];
@@ -181,18 +194,19 @@ struct DualEpi {
 // 2 A x' - A x, and the Moreau-form dual prox
 // y' = max(v - s u, 0) + min(v - s l, 0); accumulates sum dy^2 / Sigma and
 // sum dy * (A dx).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_dual(Orient<T> a, RowData<T> rd, Iterates<T> it,
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_dual(Orient<V> a, RowData<T> rd, Iterates<T> it,
                                                    const DevState* __restrict__ states,
                                                    double* __restrict__ part_d, double* __restrict__ part_i) {
+    const int4 bm = a.meta[blockIdx.x];
     const DevState& st = states[a.binst[blockIdx.x]];
-    if (st.done || st.phase != PhaseOf<T>::value) return;
+    if (st.done || st.phase != PhaseOf<V, T>::value) return;
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     __shared__ double sh[2 * kWaves];
     const int p = st.parity;
     DualEpi<T> epi{rd, it.y[p], it.ax[p], it.y[p ^ 1], it.ax[p ^ 1], static_cast<T>(st.eta_trial * st.omega)};
-    row_block(a, it.x[p ^ 1], lds, shT, epi);
+    row_block(a, bm, it.x[p ^ 1], lds, shT, epi);
     double v[2] = {epi.mov, epi.inter};
     block_sums<2>(v, sh);
     if (threadIdx.x == 0) {
@@ -331,7 +345,7 @@ __global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict
                                                       const Seg* __restrict__ segs, int ncs) {
     const Seg s = segs[blockIdx.x];
     const DevState& st = states[s.inst];
-    if (!st.avg_pending || st.phase != PhaseOf<T>::value) return;
+    if (!st.avg_pending || !iterates_of<T>(st.phase)) return;
     const T f = static_cast<T>(st.avg_frac);
     const int p = st.parity;
     T* __restrict__ avg = static_cast<int>(blockIdx.x) < ncs ? it.avg_x : it.avg_y;
@@ -356,22 +370,23 @@ __global__ __launch_bounds__(kThreads) void k_spmv(Orient<T> M, const T* __restr
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     StoreEpi<T> epi{out};
-    row_block(M, v, lds, shT, epi);
+    row_block(M, M.meta[blockIdx.x], v, lds, shT, epi);
 }
 
-// A x for the current iterate of the instances with flag != 0
-// (refreshActivity, pdhg_kernel.cpp:40-42).
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_refresh(Orient<T> a, Iterates<T> it,
+// A x for the current iterate of the instances with flag != 0 whose phase
+// uses values V and iterates T (refreshActivity, pdhg_kernel.cpp:40-42).
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_refresh(Orient<V> a, Iterates<T> it,
                                                       const DevState* __restrict__ states,
                                                       const int32_t* __restrict__ flag) {
+    const int4 bm = a.meta[blockIdx.x];
     const int k = a.binst[blockIdx.x];
-    if (!flag[k]) return;
+    if (!flag[k] || states[k].phase != PhaseOf<V, T>::value) return;
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     const int p = states[k].parity;
     StoreEpi<T> epi{it.ax[p]};
-    row_block(a, it.x[p], lds, shT, epi);
+    row_block(a, bm, it.x[p], lds, shT, epi);
 }
 
 // Working -> original coordinates for the instances with mode[k] != 0:
@@ -699,7 +714,7 @@ __global__ __launch_bounds__(kThreads) void k_farkas_cols(Orient<double> at, con
     __shared__ double shT[kWaves];
     __shared__ double sh[2 * kWaves];
     FarkasColEpi epi{lo, hi};
-    row_block(at, u, lds, shT, epi);
+    row_block(at, at.meta[blockIdx.x], u, lds, shT, epi);
     block_sums<2>(epi.v, sh);
     if (threadIdx.x == 0) {
         part[blockIdx.x] = epi.v[0];
@@ -768,7 +783,7 @@ __global__ __launch_bounds__(kThreads) void k_ray_rows(Orient<double> a, const i
     __shared__ double shT[kWaves];
     __shared__ double sh[kWaves];
     RayRowEpi epi{lo, hi};
-    row_block(a, proj, lds, shT, epi);
+    row_block(a, a.meta[blockIdx.x], proj, lds, shT, epi);
     block_sums<1>(epi.v, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = epi.v[0];
 }

@@ -295,7 +295,7 @@ pdlpx_options convert(const PdlpOptions& o) {
     r.initial_primal_weight = o.initialPrimalWeight;
     r.primal_weight_smoothing = o.primalWeightSmoothing;
     r.polishing_iterations = o.polishingIterations;
-    r.precision = o.precision == Precision::Fp32 ? PDLPX_FP32 : PDLPX_FP64;
+    r.precision = o.precision == Precision::Fp32 ? PDLPX_FP32 : (o.precision == Precision::Mixed ? PDLPX_MIXED : PDLPX_FP64);
     r.fp32_handoff_factor = o.fp32HandoffFactor;
     r.fp32_stall_checks = o.fp32StallChecks;
     return r;

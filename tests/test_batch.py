@@ -91,3 +91,30 @@ def test_batch_fp32_mixed_precision_reaches_fp64_tolerance():
         assert abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-6
         m = ol.evaluate(lp, o, r.primal, r.row_dual)
         assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+
+
+@pytest.mark.parametrize("handoff", [1.0, 1e9])
+def test_batch_mixed_fp32_values_fp64_iterates(handoff):
+    """FP32 matrix values with FP64 iterates (PDLPX_MIXED): every instance
+    reaches the FP64 tolerance as scored by the oracle on the FP64 matrix.
+    handoff=1e9 switches every instance to FP64 values in place at its first
+    check, which exercises the in-place hand-off (no restart)."""
+    from optimisation_solver_amd import zipf_lp
+    lps = [box_lp(400 + 150 * i, int(2.5 * (400 + 150 * i)), 5, seed=3200 + i) for i in range(4)]
+    lps.append(zipf_lp(3000, 7500, False, seed=3300))
+    o = _opts(1e-6, precision=2, fp32_handoff_factor=handoff)
+    with Engine() as e:
+        e.load_batch(lps, precision=2)
+        res = e.solve_batch(o)
+    for lp, r in zip(lps, res):
+        cpu = ol.solve(lp, _opts(1e-6))
+        print(f"  mixed batch (handoff {handoff:g}) {lp.name}: {r.status} its={r.iterations} "
+              f"fp32_value_its={r.diagnostics['fp32_iterations']} obj={r.primal_objective:.10g} "
+              f"cpu {cpu.status_name} its={cpu.iterations} obj={cpu.primal_objective:.10g}", flush=True)
+        assert r.status == cpu.status_name == "optimal"
+        assert r.diagnostics["fp32_iterations"] > 0
+        if handoff > 1:
+            assert r.diagnostics["fp32_iterations"] < r.iterations
+        assert abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-6
+        m = ol.evaluate(lp, o, r.primal, r.row_dual)
+        assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
