@@ -102,6 +102,10 @@ def bench_batch(args) -> int:
     eng = Engine(0)
     eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... {run} instances running, iteration {it}"), 20.0)
     eng.load_batch(lps, prec)
+    tiling = None
+    if args.autotune:
+        tiling = eng.autotune(prec, args.kernel_reps)
+        log(f"[bench] autotune: {tiling}")
     for w in range(args.warmup):
         eng.solve_batch(o)
     times, res = [], None
@@ -117,6 +121,7 @@ def bench_batch(args) -> int:
     sec = sum(times) / len(times)
     per = np.array([r.solve_time_seconds for r in res])
     ms_p, ms_d = eng.time_trial(prec, args.kernel_reps)
+    eng_tiling = eng.tiling()
     m_tot, n_tot = sum(lp.m for lp in lps), sum(lp.n for lp in lps)
 
     class _Sz:
@@ -167,7 +172,8 @@ def bench_batch(args) -> int:
                    "solves_per_s": round(len(lps) / sec, 2), "p50_instance_s": round(float(np.median(per)), 4),
                    "p99_instance_s": round(float(np.percentile(per, 99)), 4),
                    "iterations_p50": int(np.median([r.iterations for r in res])),
-                   "iterations_max": int(max(r.iterations for r in res))},
+                   "iterations_max": int(max(r.iterations for r in res)),
+                   "tiling": tiling or eng_tiling},
         "roofline": roof, "cpu_baseline": cpu, "parity": {"worst_oracle_kkt_of_16": worst},
     }
     if cpu:
@@ -189,6 +195,9 @@ def main() -> int:
                     help="PDHG arithmetic: fp64 (reference), fp32 (FP32 values and iterates), mixed (FP32 "
                          "matrix values, FP64 iterates); default mixed for --config 2 and 3, fp64 otherwise")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the candidate SpMV row-block plans after loading and keep the fastest "
+                         "(one-time, outside the timed region; reported under config.tiling)")
     ap.add_argument("--eager", action="store_true",
                     help="launch trial kernels eagerly instead of through the captured hipGraph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
@@ -228,6 +237,12 @@ def main() -> int:
     eng = Engine(0)
     eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... iteration {it}, best KKT {kkt:.3e}"), 20.0)
     eng.load(lp, prec)
+    tiling = None
+    if args.autotune:
+        t = time.perf_counter()
+        tiling = eng.autotune(prec, args.kernel_reps)
+        tiling["tune_s"] = round(time.perf_counter() - t, 3)
+        log(f"[bench] autotune: {tiling}")
     for w in range(args.warmup):
         r = eng.solve(o)
         log(f"[bench] warmup {w}: {r.status} its={r.iterations} t={r.solve_time_seconds:.3f}s")
@@ -266,6 +281,7 @@ def main() -> int:
               "oracle_dual_residual": ev["dual_residual"], "oracle_gap": ev["relative_gap"],
               "objective": res.primal_objective}
 
+    eng_tiling = eng.tiling()
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -283,7 +299,8 @@ def main() -> int:
                    "precision": pname, "restarts": res.restart_count,
                    "fp32_iterations": res.diagnostics["fp32_iterations"],
                    "device_loop_s": round(res.diagnostics["device_loop_seconds"], 4),
-                   "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name},
+                   "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name,
+                   "tiling": tiling or eng_tiling},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
         "capped": args.iteration_limit < 10_000_000,
     }

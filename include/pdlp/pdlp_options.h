@@ -61,9 +61,13 @@ struct PdlpOptions {
     // ---- MI355X engine ----
     Precision precision = Precision::Fp64;
     // The FP32 phase hands over to FP64 once the best KKT score is within
-    // this factor of the tolerance, or after this many checks without progress.
+    // this factor of the tolerance (if > 1), or after max(this many,
+    // checks / 4) checks without improving the best score.
     double fp32HandoffFactor = 1.0;
     int fp32StallChecks = 50;
+    // Time the candidate SpMV row-block plans of the problem before solving
+    // and keep the fastest (the GPU counterpart of SpmvPlan::build).
+    bool autotuneSpmv = false;
     // HIP device ordinal.
     int device = 0;
 };

@@ -240,6 +240,34 @@ typedef void (*pdlpx_progress_fn)(void* user, int32_t running, int64_t iteration
 int pdlpx_set_progress(pdlpx_ctx* ctx, pdlpx_progress_fn fn, void* user, double min_interval_seconds);
 
 /*
+ * SpMV tiling of the loaded problem.  Each matrix orientation (CSR rows of A
+ * for A x, CSC columns for A^T y) is cut into row blocks of at most
+ * *_tile_nnz nonzeros (32..2048; a longer row gets a block of its own) and
+ * *_tile_rows rows (1..256); one workgroup per block.  Defaults 2048 / 256.
+ * pdlpx_autotune times SpMVs of every candidate plan (nonzeros 256..2048 x
+ * rows 64..256) in `precision` with `reps` launches each (0 = 20) and keeps
+ * the fastest per orientation (the current plan unless beaten by 2%); it
+ * overwrites the device work buffers, so run it before a solve.  The plan
+ * stays with the loaded problem; any plan gives the same products up to
+ * summation order.  This replaces the reference's SpmvPlan::build /
+ * balancedSplit (src/sparse_matrix.cpp:13-74, include/pdlp/sparse_matrix.h:27-44),
+ * which chooses nnz-balanced chunks for CPU threads.
+ */
+typedef struct pdlpx_tiling {
+    int32_t row_tile_nnz;     /* CSR orientation (A x) */
+    int32_t row_tile_rows;
+    int32_t col_tile_nnz;     /* CSC orientation (A^T y) */
+    int32_t col_tile_rows;
+    double row_ms;            /* autotune: SpMV ms per launch with the chosen plan */
+    double col_ms;
+    double row_ms_default;    /* autotune: same with the plan in force before tuning */
+    double col_ms_default;
+} pdlpx_tiling;
+int pdlpx_autotune(pdlpx_ctx* ctx, int32_t precision, int32_t reps, pdlpx_tiling* out);
+int pdlpx_set_tiling(pdlpx_ctx* ctx, const pdlpx_tiling* tiling);
+int pdlpx_get_tiling(pdlpx_ctx* ctx, pdlpx_tiling* out);
+
+/*
  * pdlpx_load_batch followed by pdlpx_solve_loaded without warm starts.
  * Per-instance outputs: summaries[i]; x_out[i] (cols_i) and y_out[i] (rows_i)
  * may be NULL.  Options apply to every instance.

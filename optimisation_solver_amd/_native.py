@@ -75,6 +75,16 @@ class Summary(C.Structure):
         return out
 
 
+class Tiling(C.Structure):
+    _fields_ = [
+        ("row_tile_nnz", i32), ("row_tile_rows", i32), ("col_tile_nnz", i32), ("col_tile_rows", i32),
+        ("row_ms", f64), ("col_ms", f64), ("row_ms_default", f64), ("col_ms_default", f64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int64, C.c_double)
 
 _ENGINE = None
@@ -108,6 +118,9 @@ def engine() -> C.CDLL:
         lib.pdlpx_time_trial.argtypes = [vp, i32, i32, pf64, pf64]
         lib.pdlpx_solve_batch.argtypes = [vp, i32, C.POINTER(Problem), C.POINTER(Options),
                                           C.POINTER(Summary), vp, vp]
+        lib.pdlpx_autotune.argtypes = [vp, i32, i32, C.POINTER(Tiling)]
+        lib.pdlpx_set_tiling.argtypes = [vp, C.POINTER(Tiling)]
+        lib.pdlpx_get_tiling.argtypes = [vp, C.POINTER(Tiling)]
         _ENGINE = lib
     return _ENGINE
 

@@ -78,11 +78,12 @@ int gridFor(int64_t n, int threads = kThreads) {
     return static_cast<int>(std::max<int64_t>(1, (n + threads - 1) / threads));
 }
 
-// Row blocks of one orientation: at most kTileNnz nonzeros and kTileRows rows
-// each, never straddling two instances; a longer row gets a block of its own.
-// ibeg[k] = first block of instance k.
-void partitionRows(const std::vector<int32_t>& ptr, const std::vector<int32_t>& instStart,
-                   std::vector<int32_t>& blk, std::vector<int32_t>& binst, std::vector<int32_t>& ibeg) {
+// Row blocks of one orientation: at most tileNnz (<= kTileNnz) nonzeros and
+// tileRows (<= kTileRows) rows each, never straddling two instances; a longer
+// row gets a block of its own.  ibeg[k] = first block of instance k.
+void partitionRows(const std::vector<int32_t>& ptr, const std::vector<int32_t>& instStart, int tileNnz,
+                   int tileRows, std::vector<int32_t>& blk, std::vector<int32_t>& binst,
+                   std::vector<int32_t>& ibeg) {
     const int B = static_cast<int>(instStart.size()) - 1;
     blk.assign(1, 0);
     binst.clear();
@@ -94,10 +95,10 @@ void partitionRows(const std::vector<int32_t>& ptr, const std::vector<int32_t>& 
         while (r < end) {
             const int start = r;
             const int32_t base = ptr[r];
-            if (ptr[r + 1] - base > kTileNnz) {
+            if (ptr[r + 1] - base > tileNnz) {
                 ++r;
             } else {
-                while (r < end && r - start < kTileRows && ptr[r + 1] - base <= kTileNnz) ++r;
+                while (r < end && r - start < tileRows && ptr[r + 1] - base <= tileNnz) ++r;
             }
             blk.push_back(r);
             binst.push_back(k);
@@ -137,9 +138,25 @@ struct OrientBuf {
     DBuf<int4> meta;    // per block: first row, end row, first nonzero, end nonzero
     DBuf<double> val;   // original values
     int32_t nblk = 0, nrows = 0;
+    std::vector<int32_t> hptr;   // host copy of ptr, for re-planning
+    int tileNnz = kTileNnz, tileRows = kTileRows;
     template <typename T>
     Orient<T> view(const T* values) const {
         return Orient<T>{ptr.p, idx.p, values, meta.p, binst.p, nblk, nrows};
+    }
+    // Row-block plan for the given limits (blocks, owners, per-instance starts).
+    void plan(const std::vector<int32_t>& instStart, int nnzLimit, int rowLimit, hipStream_t s) {
+        std::vector<int32_t> blk, bi, ib;
+        partitionRows(hptr, instStart, nnzLimit, rowLimit, blk, bi, ib);
+        nblk = static_cast<int32_t>(bi.size());
+        std::vector<int4> mt(nblk);
+        for (int b = 0; b < nblk; ++b) mt[b] = make_int4(blk[b], blk[b + 1], hptr[blk[b]], hptr[blk[b + 1]]);
+        meta.upload(mt.data(), mt.size(), s);
+        binst.upload(bi.data(), bi.size(), s);
+        ibeg.upload(ib.data(), ib.size(), s);
+        tileNnz = nnzLimit;
+        tileRows = rowLimit;
+        HIPCHK(hipStreamSynchronize(s));   // the local staging vectors die here
     }
 };
 
@@ -825,8 +842,8 @@ std::vector<double> Run::powerIteration(std::vector<char>& converged) {
     std::vector<double> factor(B, 1.0);
     for (int it = 0; it < budget; ++it) {
         if (std::all_of(stopped.begin(), stopped.end(), [](char c) { return c != 0; })) break;
-        k_spmv<double><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.w64.vr.p), dx, dy);
-        k_spmv<double><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.w64.vc.p), dy, dz);
+        k_spmv<double, double><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<double>(X.w64.vr.p), dx, dy);
+        k_spmv<double, double><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<double>(X.w64.vc.p), dy, dz);
         k_dot_seg<<<X.ncs, kThreads, 0, s>>>(X.segs.p, dx, dz, X.part_seg.p);
         k_dot_seg<<<X.ncs, kThreads, 0, s>>>(X.segs.p, dz, dz, X.part_seg2.p);
         HIPCHK(hipGetLastError());
@@ -1355,19 +1372,22 @@ void Run::go(const double* const* warmX, const double* const* warmY, pdlpx_summa
 // ---------------------------------------------------------------------------
 void uploadOrient(OrientBuf& ob, const std::vector<int32_t>& ptr, const std::vector<int32_t>& instStart,
                   const std::vector<int32_t>& idx, const std::vector<double>& val, hipStream_t s) {
-    std::vector<int32_t> blk, binst, ibeg;
-    partitionRows(ptr, instStart, blk, binst, ibeg);
     ob.nrows = static_cast<int32_t>(ptr.size()) - 1;
-    ob.nblk = static_cast<int32_t>(binst.size());
-    std::vector<int4> meta(ob.nblk);
-    for (int b = 0; b < ob.nblk; ++b) meta[b] = make_int4(blk[b], blk[b + 1], ptr[blk[b]], ptr[blk[b + 1]]);
+    ob.hptr = ptr;
+    ob.plan(instStart, kTileNnz, kTileRows, s);
     ob.ptr.upload(ptr.data(), ptr.size(), s);
-    ob.meta.upload(meta.data(), meta.size(), s);
-    ob.binst.upload(binst.data(), binst.size(), s);
-    ob.ibeg.upload(ibeg.data(), ibeg.size(), s);
     ob.idx.upload(idx.data(), idx.size(), s);
     ob.val.upload(val.data(), val.size(), s);
     HIPCHK(hipStreamSynchronize(s));   // the local staging vectors die here
+}
+
+// Per-block partial sums of every row-block kernel (sized by the plans).
+void allocPartials(pdlpx_ctx& X) {
+    X.part_p.alloc(X.AT.nblk);
+    X.part_d.alloc(X.A.nblk);
+    X.part_i.alloc(X.A.nblk);
+    X.part_rows.alloc(6 * size_t(X.A.nblk));
+    X.part_cols.alloc(6 * size_t(X.AT.nblk));
 }
 
 void allocWork(pdlpx_ctx& X, bool fp32) {
@@ -1381,11 +1401,7 @@ void allocWork(pdlpx_ctx& X, bool fp32) {
     for (auto* b : {&w.c, &w.lo, &w.hi, &w.tau}) b->alloc(n);
     for (auto* b : {&w.rlo, &w.rhi, &w.sig}) b->alloc(m);
     X.it64.alloc(n, m);
-    X.part_p.alloc(X.AT.nblk);
-    X.part_d.alloc(X.A.nblk);
-    X.part_i.alloc(X.A.nblk);
-    X.part_rows.alloc(6 * size_t(X.A.nblk));
-    X.part_cols.alloc(6 * size_t(X.AT.nblk));
+    allocPartials(X);
     const size_t ns = size_t(X.ncs + X.nrs);
     X.part_seg.alloc(2 * ns);
     X.part_seg2.alloc(2 * ns);
@@ -1727,7 +1743,7 @@ int pdlpx_spmv(pdlpx_ctx* X, int32_t transpose, const double* x, double* y) {
         if (nin) HIPCHK(hipMemcpyAsync(din.p, hin.data(), sizeof(double) * nin, hipMemcpyHostToDevice, s));
         const OrientBuf& M = transpose ? X->AT : X->A;
         if (nout) {
-            k_spmv<double><<<M.nblk, kThreads, 0, s>>>(M.view<double>(M.val.p), din.p, dout.p);
+            k_spmv<double, double><<<M.nblk, kThreads, 0, s>>>(M.view<double>(M.val.p), din.p, dout.p);
             HIPCHK(hipGetLastError());
             HIPCHK(hipMemcpyAsync(hout.data(), dout.p, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
         }
@@ -1741,9 +1757,148 @@ int pdlpx_spmv_device(pdlpx_ctx* X, int32_t transpose, const void* x, void* y) {
     return guarded(X, [&] {
         if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
         const OrientBuf& M = transpose ? X->AT : X->A;
-        k_spmv<double><<<M.nblk, kThreads, 0, X->stream>>>(M.view<double>(M.val.p), static_cast<const double*>(x),
-                                                            static_cast<double*>(y));
+        k_spmv<double, double><<<M.nblk, kThreads, 0, X->stream>>>(M.view<double>(M.val.p),
+                                                                    static_cast<const double*>(x),
+                                                                    static_cast<double*>(y));
         HIPCHK(hipGetLastError());
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// SpMV tiling: the row-block limits of each orientation, chosen by timing.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kTuneNnz[] = {256, 512, 1024, 2048};   // candidate nonzero limits per block
+constexpr int kTuneRows[] = {64, 128, 256};          // candidate row limits per block
+
+template <typename V, typename T>
+double timeSpmv(pdlpx_ctx& X, const OrientBuf& M, const V* vals, const T* in, T* out, int reps) {
+    hipStream_t s = X.stream;
+    auto launch = [&] { k_spmv<V, T><<<M.nblk, kThreads, 0, s>>>(M.view<V>(vals), in, out); };
+    launch();
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(X.ev0, s));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHK(hipEventRecord(X.ev1, s));
+    HIPCHK(hipEventSynchronize(X.ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, X.ev0, X.ev1));
+    return ms / reps;
+}
+
+// Times every candidate plan of one orientation with SpMVs in values V and
+// vectors T; keeps the fastest unless it beats the current plan by under 2%.
+template <typename V, typename T>
+void tuneOrient(pdlpx_ctx& X, OrientBuf& M, const std::vector<int32_t>& instStart, const V* vals, const T* in,
+                T* out, int reps, int32_t* nnzOut, int32_t* rowsOut, double* msOut, double* msDefault) {
+    hipStream_t s = X.stream;
+    const int n0 = M.tileNnz, r0 = M.tileRows;
+    const double base = timeSpmv<V, T>(X, M, vals, in, out, reps);
+    double best = base;
+    int bn = n0, br = r0;
+    for (int tn : kTuneNnz)
+        for (int tr : kTuneRows) {
+            if (tn == n0 && tr == r0) continue;
+            M.plan(instStart, tn, tr, s);
+            const double t = timeSpmv<V, T>(X, M, vals, in, out, reps);
+            if (t < best) {
+                best = t;
+                bn = tn;
+                br = tr;
+            }
+        }
+    if (best > 0.98 * base) {
+        bn = n0;
+        br = r0;
+        best = base;
+    }
+    M.plan(instStart, bn, br, s);
+    *nnzOut = bn;
+    *rowsOut = br;
+    *msOut = best;
+    *msDefault = base;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdlpx_autotune(pdlpx_ctx* X, int32_t precision, int32_t reps, pdlpx_tiling* out) {
+    if (precision < PDLPX_FP64 || precision > PDLPX_MIXED) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        const int R = reps > 0 ? reps : 20;
+        hipStream_t s = X->stream;
+        if (precision != PDLPX_FP64 && !X->have32) allocWork(*X, true);
+        X->dropGraphs();
+        pdlpx_tiling t{};
+        const int n = X->n, m = X->m;
+        if (precision == PDLPX_FP64) {
+            k_fill<double><<<gridFor(n), kThreads, 0, s>>>(n, X->xo0.p, 1.0);
+            k_fill<double><<<gridFor(m), kThreads, 0, s>>>(m, X->yo1.p, 1.0);
+            tuneOrient<double, double>(*X, X->A, X->rowOff, X->A.val.p, X->xo0.p, X->yo0.p, R, &t.row_tile_nnz,
+                                       &t.row_tile_rows, &t.row_ms, &t.row_ms_default);
+            tuneOrient<double, double>(*X, X->AT, X->colOff, X->AT.val.p, X->yo1.p, X->xo1.p, R, &t.col_tile_nnz,
+                                       &t.col_tile_rows, &t.col_ms, &t.col_ms_default);
+        } else {
+            // FP32 value copies of the unscaled matrix (a solve's set-up
+            // overwrites them with the scaled values)
+            const int64_t nz = X->nnz;
+            k_convert<float, double><<<gridFor(nz), kThreads, 0, s>>>(nz, X->A.val.p, X->w32.vr.p);
+            k_convert<float, double><<<gridFor(nz), kThreads, 0, s>>>(nz, X->AT.val.p, X->w32.vc.p);
+            if (precision == PDLPX_MIXED) {
+                k_fill<double><<<gridFor(n), kThreads, 0, s>>>(n, X->xo0.p, 1.0);
+                k_fill<double><<<gridFor(m), kThreads, 0, s>>>(m, X->yo1.p, 1.0);
+                tuneOrient<float, double>(*X, X->A, X->rowOff, X->w32.vr.p, X->xo0.p, X->yo0.p, R, &t.row_tile_nnz,
+                                          &t.row_tile_rows, &t.row_ms, &t.row_ms_default);
+                tuneOrient<float, double>(*X, X->AT, X->colOff, X->w32.vc.p, X->yo1.p, X->xo1.p, R,
+                                          &t.col_tile_nnz, &t.col_tile_rows, &t.col_ms, &t.col_ms_default);
+            } else {
+                IterBuf<float>& F = X->it32;
+                k_fill<float><<<gridFor(n), kThreads, 0, s>>>(n, F.x[0].p, 1.f);
+                k_fill<float><<<gridFor(m), kThreads, 0, s>>>(m, F.y[0].p, 1.f);
+                tuneOrient<float, float>(*X, X->A, X->rowOff, X->w32.vr.p, F.x[0].p, F.ax[0].p, R, &t.row_tile_nnz,
+                                         &t.row_tile_rows, &t.row_ms, &t.row_ms_default);
+                tuneOrient<float, float>(*X, X->AT, X->colOff, X->w32.vc.p, F.y[0].p, F.x[1].p, R, &t.col_tile_nnz,
+                                         &t.col_tile_rows, &t.col_ms, &t.col_ms_default);
+            }
+        }
+        allocPartials(*X);
+        HIPCHK(hipStreamSynchronize(s));
+        if (out) *out = t;
+    });
+}
+
+int pdlpx_set_tiling(pdlpx_ctx* X, const pdlpx_tiling* t) {
+    if (!t) return PDLPX_E_INVALID;
+    auto ok = [](int nz, int rows) { return nz >= 32 && nz <= kTileNnz && rows >= 1 && rows <= kTileRows; };
+    if (!ok(t->row_tile_nnz, t->row_tile_rows) || !ok(t->col_tile_nnz, t->col_tile_rows)) {
+        if (X) X->error = "tile limits out of range: nonzeros in [32, " + std::to_string(kTileNnz) +
+                          "], rows in [1, " + std::to_string(kTileRows) + "]";
+        return PDLPX_E_INVALID;
+    }
+    return guarded(X, [&] {
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        X->dropGraphs();
+        X->A.plan(X->rowOff, t->row_tile_nnz, t->row_tile_rows, X->stream);
+        X->AT.plan(X->colOff, t->col_tile_nnz, t->col_tile_rows, X->stream);
+        allocPartials(*X);
+    });
+}
+
+int pdlpx_get_tiling(pdlpx_ctx* X, pdlpx_tiling* out) {
+    if (!out) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        pdlpx_tiling t{};
+        t.row_tile_nnz = X->A.tileNnz;
+        t.row_tile_rows = X->A.tileRows;
+        t.col_tile_nnz = X->AT.tileNnz;
+        t.col_tile_rows = X->AT.tileRows;
+        *out = t;
     });
 }
 

@@ -365,8 +365,8 @@ struct StoreEpi {
     __device__ void apply(int r, T s, const Pre&) { out[r] = s; }
 };
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void k_spmv(Orient<T> M, const T* __restrict__ v, T* __restrict__ out) {
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_spmv(Orient<V> M, const T* __restrict__ v, T* __restrict__ out) {
     __shared__ T lds[kTileNnz];
     __shared__ T shT[kWaves];
     StoreEpi<T> epi{out};

@@ -336,6 +336,7 @@ PdlpResult runSolve(const CompiledLp& problem, const PdlpOptions& options, const
     const pdlpx_problem p = view(problem);
     const pdlpx_options o = convert(options);
     int rc = pdlpx_load(c.ctx, &p, o.precision);
+    if (rc == PDLPX_OK && options.autotuneSpmv) rc = pdlpx_autotune(c.ctx, o.precision, 0, nullptr);
     if (rc != PDLPX_OK)
         return failure(rc == PDLPX_E_NOMEM ? PdlpStatus::InvalidProblem : PdlpStatus::NumericalFailure, c.error(),
                        seconds());

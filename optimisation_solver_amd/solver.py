@@ -149,6 +149,22 @@ class Engine:
                     "pdlpx_time_trial")
         return a.value, b.value
 
+    def autotune(self, precision: int = _native.FP64, reps: int = 20) -> dict:
+        """Times SpMVs of every candidate row-block plan of each orientation
+        and keeps the fastest (pdlpx_autotune).  Run before a solve."""
+        t = _native.Tiling()
+        self._check(self.lib.pdlpx_autotune(self.ctx, precision, reps, C.byref(t)), "pdlpx_autotune")
+        return t.as_dict()
+
+    def set_tiling(self, row_tile_nnz: int, row_tile_rows: int, col_tile_nnz: int, col_tile_rows: int) -> None:
+        t = _native.Tiling(row_tile_nnz, row_tile_rows, col_tile_nnz, col_tile_rows)
+        self._check(self.lib.pdlpx_set_tiling(self.ctx, C.byref(t)), "pdlpx_set_tiling")
+
+    def tiling(self) -> dict:
+        t = _native.Tiling()
+        self._check(self.lib.pdlpx_get_tiling(self.ctx, C.byref(t)), "pdlpx_get_tiling")
+        return {k: v for k, v in t.as_dict().items() if "ms" not in k}
+
     def set_progress(self, fn, min_interval: float = 10.0) -> None:
         """fn(running_instances, iteration, best_kkt) is called from the solve at
         termination checks, at most every `min_interval` seconds."""

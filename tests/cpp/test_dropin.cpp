@@ -340,6 +340,19 @@ void fp32Mode() {
     near(f.primalObjective, d.primalObjective, 1e-6 * (1 + std::abs(d.primalObjective)), "FP32 path objective");
 }
 
+void mixedAndAutotuned() {
+    auto lp = plantedLp(300, 420, 23u);
+    pdlp::PdlpOptions o;
+    o.primalTolerance = o.dualTolerance = o.gapTolerance = 1e-7;
+    auto d = pdlp::PdlpSolver{}.solve(lp, o);
+    o.precision = pdlp::Precision::Mixed;
+    o.autotuneSpmv = true;
+    auto f = pdlp::PdlpSolver{}.solve(lp, o);
+    expect(d.status == pdlp::PdlpStatus::Optimal && f.status == pdlp::PdlpStatus::Optimal, "both optimal");
+    near(f.primalObjective, d.primalObjective, 1e-6 * (1 + std::abs(d.primalObjective)),
+         "mixed precision, tuned plan objective");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -350,6 +363,7 @@ int main(int argc, char** argv) {
         {"degenerate", degenerate}, {"invalidRejected", invalidRejected}, {"timeLimit", timeLimit},
         {"linesearchAndPolicies", linesearchAndPolicies}, {"rays", rays}, {"feasibleNotFlagged", feasibleNotFlagged},
         {"warmStartNeverWorse", warmStartNeverWorse}, {"fp32Mode", fp32Mode},
+        {"mixedAndAutotuned", mixedAndAutotuned},
     };
     int failures = 0;
     for (const auto& [name, fn] : tests) {

@@ -45,7 +45,7 @@ def _struct_fields(name):
 
 def test_struct_layouts_match_header():
     for cname, py in [("pdlpx_problem", _native.Problem), ("pdlpx_options", _native.Options),
-                      ("pdlpx_summary", _native.Summary)]:
+                      ("pdlpx_summary", _native.Summary), ("pdlpx_tiling", _native.Tiling)]:
         assert _struct_fields(cname) == [f[0] for f in py._fields_], cname
 
 
