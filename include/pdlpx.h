@@ -240,6 +240,19 @@ typedef void (*pdlpx_progress_fn)(void* user, int32_t running, int64_t iteration
 int pdlpx_set_progress(pdlpx_ctx* ctx, pdlpx_progress_fn fn, void* user, double min_interval_seconds);
 
 /*
+ * Replaces the objective and/or the row bounds of loaded instance `index`
+ * (caller numbering of the load); the matrix and the variable bounds stay.
+ * For parametric families of LPs that share A (solved again and again with
+ * new costs and right-hand sides) without re-uploading the matrix.  NULL
+ * leaves a vector unchanged; row_lower and row_upper go together.  Checked
+ * like pdlpx_load (finite costs, no NaN or crossed row bounds); an invalid
+ * update returns PDLPX_E_INVALID and leaves the instance unchanged.  No
+ * reference counterpart (the reference rebuilds a CompiledLp per solve).
+ */
+int pdlpx_update_vectors(pdlpx_ctx* ctx, int32_t index, const double* objective,
+                         const double* row_lower, const double* row_upper);
+
+/*
  * SpMV tiling of the loaded problem.  Each matrix orientation (CSR rows of A
  * for A x, CSC columns for A^T y) is cut into row blocks of at most
  * *_tile_nnz nonzeros (32..2048; a longer row gets a block of its own) and

@@ -1,0 +1,84 @@
+/*
+ * pdlpx_mlp.h -- C ABI of the warm-start network: a ReLU multilayer
+ * perceptron trained and evaluated on the GPU by hand-written gfx950 kernels
+ * (BF16 operands on the MFMA matrix cores, FP32 accumulation, FP32 master
+ * weights, Adam).  It maps an LP family's parameter vector to PCA
+ * coefficients of the primal/dual solution (the learned warm start of
+ * pdlpx_solve / pdlpx_solve_loaded).  No reference counterpart: the reference
+ * solver has no warm start and no learned component.
+ *
+ * Arithmetic (the contract the CPU oracle restates, oracle/mlp_oracle.py):
+ *   H0 = bf16(x)
+ *   Z_l = H_{l-1} bf16(W_l)^T + b_l          (FP32 accumulation)
+ *   H_l = bf16(relu(Z_l))                     for l < L;   y = Z_L (FP32)
+ *   loss = mean over batch and outputs of (y - t)^2
+ *   G_L = 2 (y - t) / (batch * d_L)
+ *   dW_l = bf16(G_l)^T H_{l-1},  db_l = sum over the batch of G_l
+ *   G_{l-1} = (bf16(G_l) bf16(W_l)) * [H_{l-1} > 0]
+ * bf16() rounds to nearest even.  W_l is d_l x d_{l-1}, row-major.
+ *
+ * Every dimension must be a multiple of 64; gradient and training batches
+ * a multiple of 64 (forward pads internally).  All pointers are host memory
+ * and borrowed for the call.  Entry points return 0 or a negative PDLPX_E_*
+ * code (pdlpx.h) and never throw.
+ */
+#ifndef PDLPX_MLP_H
+#define PDLPX_MLP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdlpx_mlp pdlpx_mlp;
+
+typedef struct pdlpx_adam {
+    double learning_rate;
+    double beta1;
+    double beta2;
+    double epsilon;
+    double final_learning_rate;   /* cosine decay from learning_rate to this over the run */
+} pdlpx_adam;
+
+/* layers = number of weight matrices; dims[0..layers]; max_batch rows per
+ * device batch.  Weights ~ U(-sqrt(6/fan_in), sqrt(6/fan_in)) from `seed`,
+ * biases 0. */
+int pdlpx_mlp_create(pdlpx_mlp** net, int32_t device, int32_t layers, const int32_t* dims, int32_t max_batch,
+                     uint64_t seed);
+void pdlpx_mlp_destroy(pdlpx_mlp* net);
+const char* pdlpx_mlp_last_error(const pdlpx_mlp* net);
+
+/* FP32 master parameters of layer l (0-based): W (dims[l+1] x dims[l]) and b (dims[l+1]). */
+int pdlpx_mlp_get_params(pdlpx_mlp* net, int32_t layer, float* W, float* b);
+int pdlpx_mlp_set_params(pdlpx_mlp* net, int32_t layer, const float* W, const float* b);
+
+/* y (count x dims[L]) = network(x (count x dims[0])); any count >= 0. */
+int pdlpx_mlp_forward(pdlpx_mlp* net, int32_t count, const float* x, float* y);
+
+/* Loss and gradients of one batch (batch a multiple of 64, <= max_batch).
+ * dW[l] / db[l] may be NULL. */
+int pdlpx_mlp_gradients(pdlpx_mlp* net, int32_t batch, const float* x, const float* t, double* loss,
+                        float* const* dW, float* const* db);
+
+/* Training set kept resident on the device: x (count x dims[0]), t (count x dims[L]). */
+int pdlpx_mlp_set_data(pdlpx_mlp* net, int32_t count, const float* x, const float* t);
+
+/* Adam over the resident set: each epoch visits floor(count / batch) batches
+ * of a seeded permutation.  epoch_loss[e] (may be NULL) = mean batch loss of
+ * epoch e; *seconds (may be NULL) = hipEvent time of the whole loop;
+ * *gemm_flops (may be NULL) = matrix-core flops executed. */
+int pdlpx_mlp_train(pdlpx_mlp* net, int32_t epochs, int32_t batch, const pdlpx_adam* adam, uint64_t seed,
+                    double* epoch_loss, double* seconds, double* gemm_flops);
+
+/* C (M x N, FP32) = A(M x K) B(K x N) on the matrix cores, A and B given in
+ * FP32 and rounded to bf16; trans_a: A stored K x M; trans_b: B stored N x K.
+ * M, N multiples of 64, K of 32.  A test and timing hook for the GEMM kernel. */
+int pdlpx_mlp_gemm(pdlpx_mlp* net, int32_t M, int32_t N, int32_t K, const float* A, int32_t trans_a,
+                   const float* B, int32_t trans_b, float* C, int32_t reps, double* ms_per_gemm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDLPX_MLP_H */

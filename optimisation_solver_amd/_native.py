@@ -85,6 +85,11 @@ class Tiling(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Adam(C.Structure):
+    _fields_ = [("learning_rate", f64), ("beta1", f64), ("beta2", f64), ("epsilon", f64),
+                ("final_learning_rate", f64)]
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int64, C.c_double)
 
 _ENGINE = None
@@ -118,6 +123,19 @@ def engine() -> C.CDLL:
         lib.pdlpx_time_trial.argtypes = [vp, i32, i32, pf64, pf64]
         lib.pdlpx_solve_batch.argtypes = [vp, i32, C.POINTER(Problem), C.POINTER(Options),
                                           C.POINTER(Summary), vp, vp]
+        lib.pdlpx_mlp_create.argtypes = [C.POINTER(vp), i32, i32, vp, i32, C.c_uint64]
+        lib.pdlpx_mlp_destroy.argtypes = [vp]
+        lib.pdlpx_mlp_destroy.restype = None
+        lib.pdlpx_mlp_last_error.argtypes = [vp]
+        lib.pdlpx_mlp_last_error.restype = C.c_char_p
+        lib.pdlpx_mlp_get_params.argtypes = [vp, i32, vp, vp]
+        lib.pdlpx_mlp_set_params.argtypes = [vp, i32, vp, vp]
+        lib.pdlpx_mlp_forward.argtypes = [vp, i32, vp, vp]
+        lib.pdlpx_mlp_gradients.argtypes = [vp, i32, vp, vp, pf64, vp, vp]
+        lib.pdlpx_mlp_set_data.argtypes = [vp, i32, vp, vp]
+        lib.pdlpx_mlp_train.argtypes = [vp, i32, i32, C.POINTER(Adam), C.c_uint64, vp, pf64, pf64]
+        lib.pdlpx_mlp_gemm.argtypes = [vp, i32, i32, i32, vp, i32, vp, i32, vp, i32, pf64]
+        lib.pdlpx_update_vectors.argtypes = [vp, i32, vp, vp, vp]
         lib.pdlpx_autotune.argtypes = [vp, i32, i32, C.POINTER(Tiling)]
         lib.pdlpx_set_tiling.argtypes = [vp, C.POINTER(Tiling)]
         lib.pdlpx_get_tiling.argtypes = [vp, C.POINTER(Tiling)]

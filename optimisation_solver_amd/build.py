@@ -1,7 +1,8 @@
 """Builds the native libraries in-tree.
 
   optimisation_solver_amd/libpdlp_amd.so   engine kernels (gfx950) + C ABI +
-                                           C++ drop-in API (hipcc)
+                                           C++ drop-in API + warm-start network
+                                           kernels (hipcc)
   optimisation_solver_amd/liblpgen.so      instance generators (g++)
   tests/cpp/test_dropin                    C++ drop-in tests (g++, links the engine)
   oracle/liboracle.so, oracle/_ref/...     CPU checker (make -C oracle)
@@ -49,9 +50,10 @@ def _wait(procs: list[subprocess.Popen]) -> None:
 def build_engine() -> list[subprocess.Popen]:
     tmp = PKG / "_build"
     tmp.mkdir(exist_ok=True)
-    objs = [tmp / "engine.o", tmp / "pdlp_api.o"]
+    objs = [tmp / "engine.o", tmp / "pdlp_api.o", tmp / "mlp.o"]
     procs = [
         _run([HIPCC, *HIP_FLAGS, "-c", CSRC / "engine.hip", "-o", objs[0]]),
+        _run([HIPCC, *HIP_FLAGS, "-c", CSRC / "mlp.hip", "-o", objs[2]]),
         _run([HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", "-x", "c++", "-c",
               CSRC / "pdlp_api.cpp", "-o", objs[1]]),
     ]
@@ -61,7 +63,7 @@ def build_engine() -> list[subprocess.Popen]:
 def link_engine() -> subprocess.Popen:
     tmp = PKG / "_build"
     return _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", ENGINE_SO,
-                 tmp / "engine.o", tmp / "pdlp_api.o"])
+                 tmp / "engine.o", tmp / "pdlp_api.o", tmp / "mlp.o"])
 
 
 def build_lpgen() -> subprocess.Popen:

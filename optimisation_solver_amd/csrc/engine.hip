@@ -1764,6 +1764,58 @@ int pdlpx_spmv_device(pdlpx_ctx* X, int32_t transpose, const void* x, void* y) {
     });
 }
 
+int pdlpx_update_vectors(pdlpx_ctx* X, int32_t index, const double* objective, const double* row_lower,
+                         const double* row_upper) {
+    if ((row_lower == nullptr) != (row_upper == nullptr)) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded) throw Failure{PDLPX_E_STATE, "no problem loaded"};
+        if (index < 0 || index >= static_cast<int>(X->info.size()) || !X->info[index].valid)
+            throw Failure{PDLPX_E_INVALID, "instance index is not a loaded, valid instance"};
+        const int k = static_cast<int>(std::find(X->dev.begin(), X->dev.end(), index) - X->dev.begin());
+        InstInfo& in = X->info[index];
+        const int R = X->rowOff[k], Cc = X->colOff[k];
+        hipStream_t s = X->stream;
+        std::vector<double> c, rl, ru;
+        double dnorm = in.dnorm, pnorm = in.pnorm;
+        if (objective) {
+            double cq = 0.0;
+            for (int j = 0; j < in.n; ++j) {
+                if (!std::isfinite(objective[j]))
+                    throw Failure{PDLPX_E_INVALID, "Invalid objective coefficient or variable bounds"};
+                cq += objective[j] * objective[j];
+            }
+            c.resize(in.n);
+            for (int q = 0; q < in.n; ++q) c[q] = objective[X->xOld[Cc + q] - Cc];
+            dnorm = 1.0 + std::sqrt(cq);
+        }
+        if (row_lower) {
+            double sq = 0.0;   // termination.cpp:15-29 in the reference's order
+            for (int i = 0; i < in.m; ++i) {
+                if (std::isnan(row_lower[i]) || std::isnan(row_upper[i]) || row_lower[i] > row_upper[i])
+                    throw Failure{PDLPX_E_INVALID, "Invalid row bounds"};
+                if (std::isfinite(row_lower[i])) sq += row_lower[i] * row_lower[i];
+                if (std::isfinite(row_upper[i]) && row_upper[i] != row_lower[i]) sq += row_upper[i] * row_upper[i];
+            }
+            rl.resize(in.m);
+            ru.resize(in.m);
+            for (int q = 0; q < in.m; ++q) {
+                const int i = X->yOld[R + q] - R;
+                rl[q] = row_lower[i];
+                ru[q] = row_upper[i];
+            }
+            pnorm = 1.0 + std::sqrt(sq);
+        }
+        in.dnorm = dnorm;   // every check passed: commit
+        in.pnorm = pnorm;
+        if (!c.empty()) HIPCHK(hipMemcpyAsync(X->c.p + Cc, c.data(), sizeof(double) * c.size(), hipMemcpyHostToDevice, s));
+        if (!rl.empty()) {
+            HIPCHK(hipMemcpyAsync(X->rlo.p + R, rl.data(), sizeof(double) * rl.size(), hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(X->rhi.p + R, ru.data(), sizeof(double) * ru.size(), hipMemcpyHostToDevice, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));   // the staging vectors die here
+    });
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,607 @@
+// Warm-start network on gfx950: BF16 matrix-core GEMMs with fused epilogues,
+// FP32 master weights and Adam.  The arithmetic contract is spelled out in
+// include/pdlpx_mlp.h; oracle/mlp_oracle.py restates it in FP64 on the CPU.
+//
+// GEMM tiling (wave64): one 256-thread workgroup per 64 x 64 output tile,
+// K in steps of 32.  Each step stages the A tile (64 rows x 32 k) and the B
+// tile (64 columns x 32 k) in LDS, both k-contiguous, so every lane reads its
+// MFMA operand (8 consecutive k of one row or column) as one 16-byte LDS
+// load.  Wave w owns the 32 x 32 quadrant (w / 2, w % 2) of the tile as 2 x 2
+// v_mfma_f32_16x16x32_bf16 blocks accumulating in FP32 registers.  Operand
+// layout of that instruction: lane l supplies row (A) / column (B) l % 16 and
+// k = 8 (l / 16) .. 8 (l / 16) + 7; it returns column l % 16, rows
+// 4 (l / 16) .. 4 (l / 16) + 3 of the 16 x 16 result.  A and B use the same
+// lane -> k assignment, so the sum over k is formed whatever order the unit
+// takes the k slices in.  The epilogue adds the bias, applies ReLU or a ReLU
+// mask and writes FP32 and/or BF16 outputs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pdlpx.h"
+#include "pdlpx_mlp.h"
+
+namespace {
+
+using bf16 = uint16_t;   // storage of a bfloat16 value
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct Failure {
+    int code;
+    std::string what;
+};
+
+#define MCHK(expr)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            throw Failure{e_ == hipErrorOutOfMemory ? PDLPX_E_NOMEM : PDLPX_E_HIP,                   \
+                          std::string("HIP: ") + hipGetErrorString(e_) + " (" #expr ")"};            \
+    } while (0)
+
+// Round to nearest even (NaN -> quiet NaN).
+__host__ __device__ inline bf16 to_bf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return 0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<bf16>(u >> 16);
+}
+__host__ __device__ inline float from_bf16(bf16 h) { return __builtin_bit_cast(float, uint32_t(h) << 16); }
+
+constexpr int kBM = 64, kBN = 64, kBK = 32;
+constexpr int kLd = kBK + 8;   // LDS row stride in bf16 (80 bytes: 16-byte aligned rows)
+constexpr int kGemmThreads = 256;
+
+struct Epi {
+    const float* bias;   // [N] or null
+    int relu;
+    const bf16* mask;    // zero the result where mask(m, n) <= 0; null: no mask
+    int ldm;
+    float* outF;         // FP32 result or null
+    bf16* outH;          // bf16 result or null
+    int ldc;
+};
+
+// C(m, n) = sum_k A(m, k) B(k, n);  A(m, k) = TA ? A[k lda + m] : A[m lda + k],
+// B(k, n) = TB ? B[n ldb + k] : B[k ldb + n].  Grid (N / 64, M / 64).
+template <int TA, int TB>
+__global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16* __restrict__ A, int lda,
+                                                       const bf16* __restrict__ B, int ldb, int K, Epi e) {
+    __shared__ __attribute__((aligned(16))) bf16 As[kBM * kLd];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[kBN * kLd];
+    const int t = threadIdx.x;
+    const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
+    const int w = t >> 6, l = t & 63;
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < K; k0 += kBK) {
+        if (!TA) {   // 64 rows x 4 chunks of 8 k
+            const int r = t >> 2, ch = t & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(A + size_t(m0 + r) * lda + k0 + 8 * ch);
+            *reinterpret_cast<uint4*>(&As[r * kLd + 8 * ch]) = v;
+        } else {     // 32 k rows x 8 chunks of 8 m, transposed into LDS
+            const int kk = t >> 3, ch = t & 7;
+            const uint4 v = *reinterpret_cast<const uint4*>(A + size_t(k0 + kk) * lda + m0 + 8 * ch);
+            const bf16* el = reinterpret_cast<const bf16*>(&v);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) As[(8 * ch + q) * kLd + kk] = el[q];
+        }
+        if (TB) {
+            const int r = t >> 2, ch = t & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(B + size_t(n0 + r) * ldb + k0 + 8 * ch);
+            *reinterpret_cast<uint4*>(&Bs[r * kLd + 8 * ch]) = v;
+        } else {
+            const int kk = t >> 3, ch = t & 7;
+            const uint4 v = *reinterpret_cast<const uint4*>(B + size_t(k0 + kk) * ldb + n0 + 8 * ch);
+            const bf16* el = reinterpret_cast<const bf16*>(&v);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[(8 * ch + q) * kLd + kk] = el[q];
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = __builtin_bit_cast(bf16x8,
+                                      *reinterpret_cast<const uint4*>(&As[(wm + 16 * i + (l & 15)) * kLd + 8 * (l >> 4)]));
+            b[i] = __builtin_bit_cast(bf16x8,
+                                      *reinterpret_cast<const uint4*>(&Bs[(wn + 16 * i + (l & 15)) * kLd + 8 * (l >> 4)]));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + wn + 16 * j + (l & 15);
+            const float bv = e.bias ? e.bias[n] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + 16 * i + 4 * (l >> 4) + r;
+                float v = acc[i][j][r] + bv;
+                if (e.relu) v = v > 0.f ? v : 0.f;
+                if (e.mask && !(from_bf16(e.mask[size_t(m) * e.ldm + n]) > 0.f)) v = 0.f;
+                if (e.outF) e.outF[size_t(m) * e.ldc + n] = v;
+                if (e.outH) e.outH[size_t(m) * e.ldc + n] = to_bf16(v);
+            }
+        }
+}
+
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ src, bf16* __restrict__ dst) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = to_bf16(src[q]);
+}
+
+__global__ void k_zero_bf16(int64_t n, bf16* __restrict__ dst) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = 0;
+}
+
+// Rows perm[first .. first + rows) of the resident set into the batch buffers.
+__global__ void k_gather(int rows, const int32_t* __restrict__ perm, int first, int d0, int dl,
+                         const bf16* __restrict__ X, const float* __restrict__ T, bf16* __restrict__ H0,
+                         float* __restrict__ Tb) {
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    const int src = perm[first + r];
+    for (int q = threadIdx.x; q < d0; q += blockDim.x) H0[size_t(r) * d0 + q] = X[size_t(src) * d0 + q];
+    for (int q = threadIdx.x; q < dl; q += blockDim.x) Tb[size_t(r) * dl + q] = T[size_t(src) * dl + q];
+}
+
+// G = scale (y - t) in FP32 and bf16; *loss = mean (y - t)^2 (one workgroup,
+// fixed-order tree, so deterministic).
+constexpr int kLossThreads = 1024;
+__global__ __launch_bounds__(kLossThreads) void k_loss_grad(int64_t n, const float* __restrict__ y,
+                                                            const float* __restrict__ t, float scale,
+                                                            float* __restrict__ G, bf16* __restrict__ Gh,
+                                                            double* __restrict__ loss) {
+    __shared__ double sh[kLossThreads / 64];
+    double s = 0.0;
+    for (int64_t q = threadIdx.x; q < n; q += kLossThreads) {
+        const float d = y[q] - t[q];
+        s += double(d) * double(d);
+        const float g = scale * d;
+        G[q] = g;
+        Gh[q] = to_bf16(g);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int i = 0; i < kLossThreads / 64; ++i) tot += sh[i];
+        *loss = tot / double(n);
+    }
+}
+
+// db[c] = sum over rows of G[r][c] (rows in order).
+__global__ void k_colsum(int rows, int cols, const float* __restrict__ G, float* __restrict__ db) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.f;
+    for (int r = 0; r < rows; ++r) s += G[size_t(r) * cols + c];
+    db[c] = s;
+}
+
+struct AdamStep {
+    float lr, b1, b2, eps, bc1, bc2;
+};
+
+// Adam with bias correction; refreshes the bf16 copy when `ph` is given.
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, AdamStep a, bf16* __restrict__ ph) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) {
+        const float gq = g[q];
+        const float mq = a.b1 * m[q] + (1.f - a.b1) * gq;
+        const float vq = a.b2 * v[q] + (1.f - a.b2) * gq * gq;
+        m[q] = mq;
+        v[q] = vq;
+        const float upd = (mq / a.bc1) / (sqrtf(vq / a.bc2) + a.eps);
+        const float pq = p[q] - a.lr * upd;
+        p[q] = pq;
+        if (ph) ph[q] = to_bf16(pq);
+    }
+}
+
+int grid1(int64_t n) { return static_cast<int>(std::min<int64_t>(4096, std::max<int64_t>(1, (n + 255) / 256))); }
+
+template <typename T>
+struct Dev {
+    T* p = nullptr;
+    size_t n = 0;
+    Dev() = default;
+    Dev(const Dev&) = delete;
+    Dev& operator=(const Dev&) = delete;
+    Dev(Dev&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    ~Dev() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t count) {
+        if (p && count <= n) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        MCHK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+        n = count;
+    }
+};
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double unit(uint64_t& s) { return double(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+}  // namespace
+
+struct pdlpx_mlp {
+    int device = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string error;
+    int L = 0, maxB = 0;
+    std::vector<int> d;
+    std::vector<Dev<float>> W, b, mW, vW, mb, vb, dW, db;   // per layer
+    std::vector<Dev<bf16>> Wh;                              // bf16 copies
+    std::vector<Dev<bf16>> H;                               // H[0..L-1]: batch x d_l
+    std::vector<Dev<float>> G;                              // G[1..L]: batch x d_l (FP32)
+    std::vector<Dev<bf16>> Gh;
+    Dev<float> Y, Tb, xin;
+    Dev<double> loss;
+    Dev<bf16> X;        // resident training inputs (bf16)
+    Dev<float> T;       // resident training targets
+    Dev<int32_t> perm;
+    int count = 0;
+    int64_t adamSteps = 0;
+};
+
+namespace {
+
+void gemm(pdlpx_mlp& N, int M, int Nc, int K, const bf16* A, int lda, bool ta, const bf16* B, int ldb, bool tb,
+          const Epi& e) {
+    if (M % kBM || Nc % kBN || K % kBK || M <= 0 || Nc <= 0 || K <= 0)
+        throw Failure{PDLPX_E_INVALID, "GEMM shape must be multiples of 64 x 64 x 32"};
+    if (lda % 8 || ldb % 8 || lda < (ta ? M : K) || ldb < (tb ? K : Nc))
+        throw Failure{PDLPX_E_INVALID, "GEMM leading dimensions too small or unaligned"};
+    const dim3 grid(Nc / kBN, M / kBM);
+    if (!ta && !tb) k_gemm<0, 0><<<grid, kGemmThreads, 0, N.s>>>(A, lda, B, ldb, K, e);
+    else if (!ta && tb) k_gemm<0, 1><<<grid, kGemmThreads, 0, N.s>>>(A, lda, B, ldb, K, e);
+    else if (ta && !tb) k_gemm<1, 0><<<grid, kGemmThreads, 0, N.s>>>(A, lda, B, ldb, K, e);
+    else k_gemm<1, 1><<<grid, kGemmThreads, 0, N.s>>>(A, lda, B, ldb, K, e);
+    MCHK(hipGetLastError());
+}
+
+Epi epi(float* outF, bf16* outH, int ldc) { return Epi{nullptr, 0, nullptr, 0, outF, outH, ldc}; }
+
+// Forward over `rows` (a multiple of 64) rows already in H[0]; output in N.Y.
+void forward(pdlpx_mlp& N, int rows) {
+    for (int l = 1; l <= N.L; ++l) {
+        Epi e = epi(nullptr, nullptr, N.d[l]);
+        e.bias = N.b[l - 1].p;
+        if (l < N.L) {
+            e.relu = 1;
+            e.outH = N.H[l].p;
+        } else {
+            e.outF = N.Y.p;
+        }
+        gemm(N, rows, N.d[l], N.d[l - 1], N.H[l - 1].p, N.d[l - 1], false, N.Wh[l - 1].p, N.d[l - 1], true, e);
+    }
+}
+
+// Loss gradient and backward pass for `rows` rows (targets in N.Tb):
+// dW, db of every layer; the loss lands in N.loss.p[slot].
+void backward(pdlpx_mlp& N, int rows, int slot) {
+    const int dl = N.d[N.L];
+    const int64_t n = int64_t(rows) * dl;
+    k_loss_grad<<<1, kLossThreads, 0, N.s>>>(n, N.Y.p, N.Tb.p, 2.f / float(n), N.G[N.L].p, N.Gh[N.L].p,
+                                              N.loss.p + slot);
+    MCHK(hipGetLastError());
+    for (int l = N.L; l >= 1; --l) {
+        // dW_l (d_l x d_{l-1}) = G_l^T H_{l-1}
+        gemm(N, N.d[l], N.d[l - 1], rows, N.Gh[l].p, N.d[l], true, N.H[l - 1].p, N.d[l - 1], false,
+             epi(N.dW[l - 1].p, nullptr, N.d[l - 1]));
+        k_colsum<<<(N.d[l] + 255) / 256, 256, 0, N.s>>>(rows, N.d[l], N.G[l].p, N.db[l - 1].p);
+        MCHK(hipGetLastError());
+        if (l > 1) {   // G_{l-1} = (G_l W_l) * [H_{l-1} > 0]
+            Epi e = epi(N.G[l - 1].p, N.Gh[l - 1].p, N.d[l - 1]);
+            e.mask = N.H[l - 1].p;
+            e.ldm = N.d[l - 1];
+            gemm(N, rows, N.d[l - 1], N.d[l], N.Gh[l].p, N.d[l], false, N.Wh[l - 1].p, N.d[l - 1], false, e);
+        }
+    }
+}
+
+double stepFlops(const pdlpx_mlp& N, int rows) {
+    double f = 0.0;
+    for (int l = 1; l <= N.L; ++l) f += 2.0 * rows * N.d[l] * N.d[l - 1] * (l > 1 ? 3.0 : 2.0);
+    return f;
+}
+
+template <typename F>
+int guardedM(pdlpx_mlp* N, F&& body) {
+    if (!N) return PDLPX_E_INVALID;
+    try {
+        MCHK(hipSetDevice(N->device));
+        body();
+        return PDLPX_OK;
+    } catch (const Failure& f) {
+        N->error = f.what;
+        return f.code;
+    } catch (const std::bad_alloc&) {
+        N->error = "host allocation failed";
+        return PDLPX_E_NOMEM;
+    } catch (...) {
+        N->error = "unexpected failure";
+        return PDLPX_E_INVALID;
+    }
+}
+
+void uploadBf16(pdlpx_mlp& N, const float* host, int64_t count, bf16* dst) {
+    N.xin.alloc(size_t(count));
+    MCHK(hipMemcpyAsync(N.xin.p, host, sizeof(float) * count, hipMemcpyHostToDevice, N.s));
+    k_to_bf16<<<grid1(count), 256, 0, N.s>>>(count, N.xin.p, dst);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(N.s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdlpx_mlp_create(pdlpx_mlp** out, int32_t device, int32_t layers, const int32_t* dims, int32_t max_batch,
+                     uint64_t seed) {
+    if (!out || layers < 1 || !dims || max_batch < 64 || max_batch % 64) return PDLPX_E_INVALID;
+    for (int l = 0; l <= layers; ++l)
+        if (dims[l] <= 0 || dims[l] % 64) return PDLPX_E_INVALID;
+    *out = nullptr;
+    pdlpx_mlp* N = new (std::nothrow) pdlpx_mlp();
+    if (!N) return PDLPX_E_NOMEM;
+    N->device = device;
+    const int rc = guardedM(N, [&] {
+        MCHK(hipStreamCreate(&N->s));
+        MCHK(hipEventCreate(&N->e0));
+        MCHK(hipEventCreate(&N->e1));
+        N->L = layers;
+        N->maxB = max_batch;
+        N->d.assign(dims, dims + layers + 1);
+        const int L = layers;
+        for (auto* v : {&N->W, &N->b, &N->mW, &N->vW, &N->mb, &N->vb, &N->dW, &N->db}) v->resize(L);
+        N->Wh.resize(L);
+        N->H.resize(L);
+        N->G.resize(L + 1);
+        N->Gh.resize(L + 1);
+        uint64_t st = seed;
+        for (int l = 0; l < L; ++l) {
+            const size_t nw = size_t(N->d[l + 1]) * N->d[l], nb = size_t(N->d[l + 1]);
+            for (auto* v : {&N->W[l], &N->mW[l], &N->vW[l], &N->dW[l]}) v->alloc(nw);
+            for (auto* v : {&N->b[l], &N->mb[l], &N->vb[l], &N->db[l]}) v->alloc(nb);
+            N->Wh[l].alloc(nw);
+            const double lim = std::sqrt(6.0 / N->d[l]);
+            std::vector<float> w(nw), z(nw, 0.f);
+            for (auto& x : w) x = static_cast<float>((2.0 * unit(st) - 1.0) * lim);
+            MCHK(hipMemcpyAsync(N->W[l].p, w.data(), sizeof(float) * nw, hipMemcpyHostToDevice, N->s));
+            MCHK(hipMemcpyAsync(N->mW[l].p, z.data(), sizeof(float) * nw, hipMemcpyHostToDevice, N->s));
+            MCHK(hipMemcpyAsync(N->vW[l].p, z.data(), sizeof(float) * nw, hipMemcpyHostToDevice, N->s));
+            MCHK(hipMemcpyAsync(N->b[l].p, z.data(), sizeof(float) * nb, hipMemcpyHostToDevice, N->s));
+            MCHK(hipMemcpyAsync(N->mb[l].p, z.data(), sizeof(float) * nb, hipMemcpyHostToDevice, N->s));
+            MCHK(hipMemcpyAsync(N->vb[l].p, z.data(), sizeof(float) * nb, hipMemcpyHostToDevice, N->s));
+            k_to_bf16<<<grid1(int64_t(nw)), 256, 0, N->s>>>(int64_t(nw), N->W[l].p, N->Wh[l].p);
+            MCHK(hipGetLastError());
+            MCHK(hipStreamSynchronize(N->s));   // staging vectors die here
+        }
+        for (int l = 0; l < L; ++l) N->H[l].alloc(size_t(max_batch) * N->d[l]);
+        for (int l = 1; l <= L; ++l) {
+            N->G[l].alloc(size_t(max_batch) * N->d[l]);
+            N->Gh[l].alloc(size_t(max_batch) * N->d[l]);
+        }
+        N->Y.alloc(size_t(max_batch) * N->d[L]);
+        N->Tb.alloc(size_t(max_batch) * N->d[L]);
+        N->loss.alloc(1);
+    });
+    if (rc != PDLPX_OK) {
+        pdlpx_mlp_destroy(N);
+        return rc;
+    }
+    *out = N;
+    return PDLPX_OK;
+}
+
+void pdlpx_mlp_destroy(pdlpx_mlp* N) {
+    if (!N) return;
+    (void)hipSetDevice(N->device);
+    if (N->s) (void)hipStreamSynchronize(N->s);
+    if (N->e0) (void)hipEventDestroy(N->e0);
+    if (N->e1) (void)hipEventDestroy(N->e1);
+    hipStream_t s = N->s;
+    delete N;   // frees the device buffers
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* pdlpx_mlp_last_error(const pdlpx_mlp* N) { return N ? N->error.c_str() : "null network"; }
+
+int pdlpx_mlp_get_params(pdlpx_mlp* N, int32_t layer, float* W, float* b) {
+    return guardedM(N, [&] {
+        if (layer < 0 || layer >= N->L) throw Failure{PDLPX_E_INVALID, "layer out of range"};
+        const size_t nw = size_t(N->d[layer + 1]) * N->d[layer], nb = size_t(N->d[layer + 1]);
+        if (W) MCHK(hipMemcpyAsync(W, N->W[layer].p, sizeof(float) * nw, hipMemcpyDeviceToHost, N->s));
+        if (b) MCHK(hipMemcpyAsync(b, N->b[layer].p, sizeof(float) * nb, hipMemcpyDeviceToHost, N->s));
+        MCHK(hipStreamSynchronize(N->s));
+    });
+}
+
+int pdlpx_mlp_set_params(pdlpx_mlp* N, int32_t layer, const float* W, const float* b) {
+    return guardedM(N, [&] {
+        if (layer < 0 || layer >= N->L) throw Failure{PDLPX_E_INVALID, "layer out of range"};
+        const size_t nw = size_t(N->d[layer + 1]) * N->d[layer], nb = size_t(N->d[layer + 1]);
+        if (W) {
+            MCHK(hipMemcpyAsync(N->W[layer].p, W, sizeof(float) * nw, hipMemcpyHostToDevice, N->s));
+            k_to_bf16<<<grid1(int64_t(nw)), 256, 0, N->s>>>(int64_t(nw), N->W[layer].p, N->Wh[layer].p);
+            MCHK(hipGetLastError());
+        }
+        if (b) MCHK(hipMemcpyAsync(N->b[layer].p, b, sizeof(float) * nb, hipMemcpyHostToDevice, N->s));
+        MCHK(hipStreamSynchronize(N->s));
+    });
+}
+
+int pdlpx_mlp_forward(pdlpx_mlp* N, int32_t count, const float* x, float* y) {
+    if (count < 0 || (count > 0 && (!x || !y))) return PDLPX_E_INVALID;
+    return guardedM(N, [&] {
+        const int d0 = N->d[0], dl = N->d[N->L];
+        for (int first = 0; first < count; first += N->maxB) {
+            const int rows = std::min(N->maxB, count - first);
+            const int padded = (rows + 63) / 64 * 64;
+            k_zero_bf16<<<grid1(int64_t(padded) * d0), 256, 0, N->s>>>(int64_t(padded) * d0, N->H[0].p);
+            MCHK(hipGetLastError());
+            uploadBf16(*N, x + size_t(first) * d0, int64_t(rows) * d0, N->H[0].p);
+            forward(*N, padded);
+            MCHK(hipMemcpyAsync(y + size_t(first) * dl, N->Y.p, sizeof(float) * size_t(rows) * dl,
+                                hipMemcpyDeviceToHost, N->s));
+            MCHK(hipStreamSynchronize(N->s));
+        }
+    });
+}
+
+int pdlpx_mlp_gradients(pdlpx_mlp* N, int32_t batch, const float* x, const float* t, double* loss,
+                        float* const* dW, float* const* db) {
+    if (!x || !t) return PDLPX_E_INVALID;
+    return guardedM(N, [&] {
+        if (batch <= 0 || batch % 64 || batch > N->maxB)
+            throw Failure{PDLPX_E_INVALID, "batch must be a positive multiple of 64 within max_batch"};
+        const int d0 = N->d[0], dl = N->d[N->L];
+        uploadBf16(*N, x, int64_t(batch) * d0, N->H[0].p);
+        MCHK(hipMemcpyAsync(N->Tb.p, t, sizeof(float) * size_t(batch) * dl, hipMemcpyHostToDevice, N->s));
+        forward(*N, batch);
+        backward(*N, batch, 0);
+        if (loss) MCHK(hipMemcpyAsync(loss, N->loss.p, sizeof(double), hipMemcpyDeviceToHost, N->s));
+        for (int l = 0; l < N->L; ++l) {
+            const size_t nw = size_t(N->d[l + 1]) * N->d[l], nb = size_t(N->d[l + 1]);
+            if (dW && dW[l]) MCHK(hipMemcpyAsync(dW[l], N->dW[l].p, sizeof(float) * nw, hipMemcpyDeviceToHost, N->s));
+            if (db && db[l]) MCHK(hipMemcpyAsync(db[l], N->db[l].p, sizeof(float) * nb, hipMemcpyDeviceToHost, N->s));
+        }
+        MCHK(hipStreamSynchronize(N->s));
+    });
+}
+
+int pdlpx_mlp_set_data(pdlpx_mlp* N, int32_t count, const float* x, const float* t) {
+    if (count < 0 || (count > 0 && (!x || !t))) return PDLPX_E_INVALID;
+    return guardedM(N, [&] {
+        const int d0 = N->d[0], dl = N->d[N->L];
+        N->X.alloc(size_t(count) * d0);
+        N->T.alloc(size_t(count) * dl);
+        N->perm.alloc(size_t(count));
+        if (count) {
+            uploadBf16(*N, x, int64_t(count) * d0, N->X.p);
+            MCHK(hipMemcpyAsync(N->T.p, t, sizeof(float) * size_t(count) * dl, hipMemcpyHostToDevice, N->s));
+        }
+        MCHK(hipStreamSynchronize(N->s));
+        N->count = count;
+    });
+}
+
+int pdlpx_mlp_train(pdlpx_mlp* N, int32_t epochs, int32_t batch, const pdlpx_adam* adam, uint64_t seed,
+                    double* epoch_loss, double* seconds, double* gemm_flops) {
+    if (!adam || epochs < 0) return PDLPX_E_INVALID;
+    return guardedM(N, [&] {
+        if (batch <= 0 || batch % 64 || batch > N->maxB)
+            throw Failure{PDLPX_E_INVALID, "batch must be a positive multiple of 64 within max_batch"};
+        const int steps = N->count / batch;
+        if (steps == 0 && epochs > 0) throw Failure{PDLPX_E_INVALID, "fewer training rows than one batch"};
+        const int d0 = N->d[0], dl = N->d[N->L];
+        N->loss.alloc(size_t(std::max(steps, 1)));
+        std::vector<int32_t> perm(N->count);
+        std::vector<double> losses(std::max(steps, 1));
+        uint64_t st = seed;
+        const double total = double(epochs) * steps;
+        double flops = 0.0, busy = 0.0;
+        for (int ep = 0; ep < epochs; ++ep) {
+            for (int i = 0; i < N->count; ++i) perm[i] = i;
+            for (int i = N->count - 1; i > 0; --i) std::swap(perm[i], perm[splitmix(st) % uint64_t(i + 1)]);
+            MCHK(hipMemcpyAsync(N->perm.p, perm.data(), sizeof(int32_t) * perm.size(), hipMemcpyHostToDevice, N->s));
+            MCHK(hipEventRecord(N->e0, N->s));
+            for (int k = 0; k < steps; ++k) {
+                k_gather<<<batch, 256, 0, N->s>>>(batch, N->perm.p, k * batch, d0, dl, N->X.p, N->T.p, N->H[0].p,
+                                                  N->Tb.p);
+                MCHK(hipGetLastError());
+                forward(*N, batch);
+                backward(*N, batch, k);
+                const double tt = double(N->adamSteps++) + 1.0;
+                const double frac = total > 0 ? (double(ep) * steps + k) / total : 0.0;
+                const double lr = adam->final_learning_rate +
+                                  0.5 * (adam->learning_rate - adam->final_learning_rate) * (1.0 + std::cos(M_PI * frac));
+                AdamStep a{float(lr), float(adam->beta1), float(adam->beta2), float(adam->epsilon),
+                           float(1.0 - std::pow(adam->beta1, tt)), float(1.0 - std::pow(adam->beta2, tt))};
+                for (int l = 0; l < N->L; ++l) {
+                    const int64_t nw = int64_t(N->d[l + 1]) * N->d[l], nb = N->d[l + 1];
+                    k_adam<<<grid1(nw), 256, 0, N->s>>>(nw, N->W[l].p, N->dW[l].p, N->mW[l].p, N->vW[l].p, a, N->Wh[l].p);
+                    k_adam<<<grid1(nb), 256, 0, N->s>>>(nb, N->b[l].p, N->db[l].p, N->mb[l].p, N->vb[l].p, a, nullptr);
+                    MCHK(hipGetLastError());
+                }
+                flops += stepFlops(*N, batch);
+            }
+            MCHK(hipEventRecord(N->e1, N->s));
+            MCHK(hipMemcpyAsync(losses.data(), N->loss.p, sizeof(double) * steps, hipMemcpyDeviceToHost, N->s));
+            MCHK(hipStreamSynchronize(N->s));
+            float ms = 0.f;
+            MCHK(hipEventElapsedTime(&ms, N->e0, N->e1));
+            busy += ms * 1e-3;
+            if (epoch_loss) {
+                double s = 0.0;
+                for (int k = 0; k < steps; ++k) s += losses[k];
+                epoch_loss[ep] = s / steps;
+            }
+        }
+        if (seconds) *seconds = busy;
+        if (gemm_flops) *gemm_flops = flops;
+    });
+}
+
+int pdlpx_mlp_gemm(pdlpx_mlp* N, int32_t M, int32_t Nc, int32_t K, const float* A, int32_t trans_a, const float* B,
+                   int32_t trans_b, float* C, int32_t reps, double* ms_per_gemm) {
+    if (!A || !B || !C) return PDLPX_E_INVALID;
+    return guardedM(N, [&] {
+        if (M <= 0 || Nc <= 0 || K <= 0 || M % 64 || Nc % 64 || K % 32)
+            throw Failure{PDLPX_E_INVALID, "GEMM shape must be multiples of 64 x 64 x 32"};
+        Dev<bf16> a, b;
+        Dev<float> c;
+        a.alloc(size_t(M) * K);
+        b.alloc(size_t(K) * Nc);
+        c.alloc(size_t(M) * Nc);
+        uploadBf16(*N, A, int64_t(M) * K, a.p);
+        uploadBf16(*N, B, int64_t(K) * Nc, b.p);
+        const int lda = trans_a ? M : K, ldb = trans_b ? K : Nc;
+        const Epi e = epi(c.p, nullptr, Nc);
+        gemm(*N, M, Nc, K, a.p, lda, trans_a != 0, b.p, ldb, trans_b != 0, e);
+        const int R = std::max(reps, 0);
+        if (R > 0) {
+            MCHK(hipEventRecord(N->e0, N->s));
+            for (int r = 0; r < R; ++r) gemm(*N, M, Nc, K, a.p, lda, trans_a != 0, b.p, ldb, trans_b != 0, e);
+            MCHK(hipEventRecord(N->e1, N->s));
+        }
+        MCHK(hipMemcpyAsync(C, c.p, sizeof(float) * size_t(M) * Nc, hipMemcpyDeviceToHost, N->s));
+        MCHK(hipStreamSynchronize(N->s));
+        if (ms_per_gemm) {
+            float ms = 0.f;
+            if (R > 0) MCHK(hipEventElapsedTime(&ms, N->e0, N->e1));
+            *ms_per_gemm = R > 0 ? ms / R : 0.0;
+        }
+    });
+}
+
+}  // extern "C"

@@ -149,6 +149,13 @@ class Engine:
                     "pdlpx_time_trial")
         return a.value, b.value
 
+    def update_vectors(self, index: int, c=None, rlo=None, rhi=None) -> None:
+        """New objective and/or row bounds for loaded instance `index`; the
+        matrix stays on the device (pdlpx_update_vectors)."""
+        keep = [None if v is None else np.ascontiguousarray(v, dtype=np.float64) for v in (c, rlo, rhi)]
+        ptr = [None if v is None else v.ctypes.data for v in keep]
+        self._check(self.lib.pdlpx_update_vectors(self.ctx, index, *ptr), "pdlpx_update_vectors")
+
     def autotune(self, precision: int = _native.FP64, reps: int = 20) -> dict:
         """Times SpMVs of every candidate row-block plan of each orientation
         and keeps the fastest (pdlpx_autotune).  Run before a solve."""

@@ -1,0 +1,115 @@
+"""Host wrapper of the warm-start network (include/pdlpx_mlp.h): every
+forward, backward and Adam step runs in libpdlp_amd.so's gfx950 kernels."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _native
+
+
+class MlpError(RuntimeError):
+    pass
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """ReLU MLP dims[0] -> ... -> dims[-1]; BF16 MFMA GEMMs, FP32 master weights."""
+
+    def __init__(self, dims, max_batch: int = 256, seed: int = 1, device: int = 0):
+        self.lib = _native.engine()
+        self.dims = [int(d) for d in dims]
+        self.max_batch = int(max_batch)
+        self.net = C.c_void_p()
+        arr = (C.c_int32 * len(self.dims))(*self.dims)
+        rc = self.lib.pdlpx_mlp_create(C.byref(self.net), device, len(self.dims) - 1, arr, self.max_batch, seed)
+        if rc != 0:
+            raise MlpError(f"pdlpx_mlp_create failed ({rc}): dims must be multiples of 64, max_batch of 64")
+
+    @property
+    def layers(self) -> int:
+        return len(self.dims) - 1
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise MlpError(f"{what} failed ({rc}): {self.lib.pdlpx_mlp_last_error(self.net).decode()}")
+
+    def params(self) -> list[tuple[np.ndarray, np.ndarray]]:
+        out = []
+        for l in range(self.layers):
+            W = np.zeros((self.dims[l + 1], self.dims[l]), np.float32)
+            b = np.zeros(self.dims[l + 1], np.float32)
+            self._check(self.lib.pdlpx_mlp_get_params(self.net, l, W.ctypes.data, b.ctypes.data), "get_params")
+            out.append((W, b))
+        return out
+
+    def set_params(self, params) -> None:
+        for l, (W, b) in enumerate(params):
+            W, b = _f32(W), _f32(b)
+            assert W.shape == (self.dims[l + 1], self.dims[l]) and b.shape == (self.dims[l + 1],)
+            self._check(self.lib.pdlpx_mlp_set_params(self.net, l, W.ctypes.data, b.ctypes.data), "set_params")
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x)
+        y = np.zeros((x.shape[0], self.dims[-1]), np.float32)
+        self._check(self.lib.pdlpx_mlp_forward(self.net, x.shape[0], x.ctypes.data, y.ctypes.data), "forward")
+        return y
+
+    def gradients(self, x, t):
+        x, t = _f32(x), _f32(t)
+        dW = [np.zeros((self.dims[l + 1], self.dims[l]), np.float32) for l in range(self.layers)]
+        db = [np.zeros(self.dims[l + 1], np.float32) for l in range(self.layers)]
+        P = C.c_void_p * self.layers
+        loss = C.c_double()
+        self._check(self.lib.pdlpx_mlp_gradients(self.net, x.shape[0], x.ctypes.data, t.ctypes.data, C.byref(loss),
+                                                 P(*[a.ctypes.data for a in dW]), P(*[a.ctypes.data for a in db])),
+                    "gradients")
+        return loss.value, dW, db
+
+    def set_data(self, x, t) -> None:
+        self._x, self._t = _f32(x), _f32(t)
+        self._check(self.lib.pdlpx_mlp_set_data(self.net, self._x.shape[0], self._x.ctypes.data, self._t.ctypes.data),
+                    "set_data")
+
+    def train(self, epochs: int, batch: int = 256, lr: float = 1e-3, final_lr: float = 1e-5, beta1: float = 0.9,
+              beta2: float = 0.999, eps: float = 1e-8, seed: int = 7) -> dict:
+        losses = np.zeros(max(epochs, 1))
+        sec, flops = C.c_double(), C.c_double()
+        a = _native.Adam(lr, beta1, beta2, eps, final_lr)
+        self._check(self.lib.pdlpx_mlp_train(self.net, epochs, batch, C.byref(a), seed, losses.ctypes.data,
+                                             C.byref(sec), C.byref(flops)), "train")
+        return {"epoch_loss": losses[:epochs], "seconds": sec.value, "gemm_flops": flops.value}
+
+    def gemm(self, A, B, trans_a: bool = False, trans_b: bool = False, reps: int = 0):
+        """C = op(A) op(B) on the matrix cores (operands rounded to bf16)."""
+        A, B = _f32(A), _f32(B)
+        M = A.shape[1] if trans_a else A.shape[0]
+        K = A.shape[0] if trans_a else A.shape[1]
+        N = B.shape[0] if trans_b else B.shape[1]
+        assert (B.shape[1] if trans_b else B.shape[0]) == K
+        Cm = np.zeros((M, N), np.float32)
+        ms = C.c_double()
+        self._check(self.lib.pdlpx_mlp_gemm(self.net, M, N, K, A.ctypes.data, int(trans_a), B.ctypes.data,
+                                            int(trans_b), Cm.ctypes.data, reps, C.byref(ms)), "gemm")
+        return Cm, ms.value
+
+    def close(self) -> None:
+        if self.net:
+            self.lib.pdlpx_mlp_destroy(self.net)
+            self.net = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
