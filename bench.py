@@ -182,14 +182,89 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_warmstart(args) -> int:
+    """Learned warm start on the parametric block-angular family: training
+    solves on the GPU, PCA, the BF16 MFMA network, then held-out instances
+    solved cold (FP64) and warm-started (FP64, and mixed precision), each to
+    1e-6; the mean training solution as a warm start is the method without
+    the network."""
+    import numpy as np
+    from optimisation_solver_amd import _native
+    from optimisation_solver_amd.warmstart import BlockFamily
+    from optimisation_solver_amd.warmstart import pipeline as P
+
+    t = time.perf_counter()
+    fam = BlockFamily()
+    log(f"[bench] family: m={fam.m} n={fam.n} nnz={fam.base.nnz} theta dim {fam.dim} in {time.perf_counter() - t:.1f}s")
+    ntr, nte, B = args.train_count, args.test_count, 64
+    train = P.generate(fam, fam.thetas(0, ntr), args.train_tolerance, batch=B, precision=_native.MIXED)
+    log(f"[bench] {ntr} training solves to {args.train_tolerance:g} in {train.seconds:.1f}s "
+        f"(iterations p50 {np.median(train.iterations):.0f})")
+    model = P.WarmStartModel(fam)
+    fit = model.fit(train, epochs=args.epochs)
+    del train
+    mlp_tflops = fit["gemm_flops"] / max(fit["seconds"], 1e-9) / 1e12
+    log(f"[bench] network: val loss {fit['best_val_loss']:.4f}, {fit['seconds']:.2f}s on device "
+        f"({mlp_tflops:.2f} TFLOP/s), PCA {fit['pca_seconds']:.1f}s, explained x {fit['explained_x']:.3f} "
+        f"y {fit['explained_y']:.3f}")
+    th = fam.thetas(100000, nte)
+    wx, wy = model.predict(th)
+    mean_x = np.repeat(model.pca_x.mean[None], nte, 0)
+    mean_y = np.repeat(model.pca_y.mean[None], nte, 0)
+    model.close()
+    res = {}
+    for tag, xs, ys, prec in (("warm_fp64", wx, wy, _native.FP64), ("warm_mixed", wx, wy, _native.MIXED),
+                              ("mean_warm_fp64", mean_x, mean_y, _native.FP64)):
+        log(f"[bench] held-out: cold FP64 vs {tag}")
+        res[tag] = P.compare(fam, th, xs, ys, tol=1e-6, batch=B, cold_precision=_native.FP64, warm_precision=prec)
+    cold = res["warm_fp64"]
+
+    def summary(c):
+        return {"iterations_p50": float(np.median(c.warm_iterations)),
+                "median_iteration_ratio": round(float(np.median(c.cold_iterations / np.maximum(c.warm_iterations, 1))), 3),
+                "median_time_ratio": round(float(np.median(c.cold_seconds / np.maximum(c.warm_seconds, 1e-9))), 3),
+                "wall_s": round(c.warm_wall, 3), "wall_ratio": round(c.cold_wall / max(c.warm_wall, 1e-9), 3),
+                "fallbacks": c.fallbacks, "worst_kkt": float(np.max(c.warm_kkt))}
+
+    per_cold = np.median(cold.cold_seconds)
+    best = res["warm_mixed"]
+    line = {
+        "metric": METRIC, "value": round(float(np.median(best.warm_seconds)), 4), "unit": "s",
+        "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(best.warm_wall / max(nte, 1) * 1e3, 3),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 values, f64 iterates (warm); bf16 network", "data": "synthetic",
+        "config": {"workload": f"c4_block_{fam.m}x{fam.n}_theta{fam.dim}", "nnz": fam.base.nnz,
+                   "train_solves": ntr, "train_tolerance": args.train_tolerance, "held_out": nte,
+                   "tolerance": 1e-6, "device_batch": B,
+                   "cold_fp64": {"iterations_p50": float(np.median(cold.cold_iterations)),
+                                 "per_instance_s_p50": round(float(per_cold), 4), "wall_s": round(cold.cold_wall, 3),
+                                 "worst_kkt": float(np.max(cold.cold_kkt))},
+                   "warm": {k: summary(v) for k, v in res.items()},
+                   "network": {"dims": [fam.dim, 512, 512, 512], "best_val_loss": fit["best_val_loss"],
+                               "val_loss_by_round": fit["val_loss"], "explained_x": fit["explained_x"],
+                               "explained_y": fit["explained_y"], "train_s": round(fit["seconds"], 3),
+                               "pca_s": round(fit["pca_seconds"], 2)}},
+        "roofline": {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(mlp_tflops / 2500.0, 5), "traffic": None, "kernel": "k_gemm (bf16 MFMA)",
+                     "note": "network training GEMMs (batch 256, 256-512 wide): latency-bound by size"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3])
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--batch-count", type=int, default=256, help="instances of the batched workload (--config 3)")
     ap.add_argument("--cpu-sample-iters", type=int, default=200)
+    ap.add_argument("--train-count", type=int, default=8000, help="training solves of the warm-start workload")
+    ap.add_argument("--test-count", type=int, default=256, help="held-out instances of the warm-start workload")
+    ap.add_argument("--train-tolerance", type=float, default=1e-4)
+    ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default=None,
                     help="PDHG arithmetic: fp64 (reference), fp32 (FP32 values and iterates), mixed (FP32 "
@@ -224,6 +299,8 @@ def main() -> int:
 
     if args.config == 3:
         return bench_batch(args)
+    if args.config == 4:
+        return bench_warmstart(args)
     cfg = CONFIGS[args.config]
     t = time.perf_counter()
     lp = cfg["make"]()

@@ -39,6 +39,7 @@ typedef struct pdlpx_adam {
     double beta2;
     double epsilon;
     double final_learning_rate;   /* cosine decay from learning_rate to this over the run */
+    double weight_decay;          /* decoupled (AdamW) decay of the weight matrices, not the biases */
 } pdlpx_adam;
 
 /* layers = number of weight matrices; dims[0..layers]; max_batch rows per

@@ -87,7 +87,7 @@ class Tiling(C.Structure):
 
 class Adam(C.Structure):
     _fields_ = [("learning_rate", f64), ("beta1", f64), ("beta2", f64), ("epsilon", f64),
-                ("final_learning_rate", f64)]
+                ("final_learning_rate", f64), ("weight_decay", f64)]
 
 
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int64, C.c_double)

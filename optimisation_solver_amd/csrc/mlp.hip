@@ -199,10 +199,11 @@ __global__ void k_colsum(int rows, int cols, const float* __restrict__ G, float*
 }
 
 struct AdamStep {
-    float lr, b1, b2, eps, bc1, bc2;
+    float lr, b1, b2, eps, bc1, bc2, wd;
 };
 
-// Adam with bias correction; refreshes the bf16 copy when `ph` is given.
+// Adam with bias correction and decoupled weight decay `wd` (AdamW);
+// refreshes the bf16 copy when `ph` is given.
 __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, AdamStep a, bf16* __restrict__ ph) {
     const int64_t stride = int64_t(gridDim.x) * blockDim.x;
@@ -213,7 +214,7 @@ __global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict
         m[q] = mq;
         v[q] = vq;
         const float upd = (mq / a.bc1) / (sqrtf(vq / a.bc2) + a.eps);
-        const float pq = p[q] - a.lr * upd;
+        const float pq = p[q] - a.lr * (upd + a.wd * p[q]);
         p[q] = pq;
         if (ph) ph[q] = to_bf16(pq);
     }
@@ -546,11 +547,14 @@ int pdlpx_mlp_train(pdlpx_mlp* N, int32_t epochs, int32_t batch, const pdlpx_ada
                 const double lr = adam->final_learning_rate +
                                   0.5 * (adam->learning_rate - adam->final_learning_rate) * (1.0 + std::cos(M_PI * frac));
                 AdamStep a{float(lr), float(adam->beta1), float(adam->beta2), float(adam->epsilon),
-                           float(1.0 - std::pow(adam->beta1, tt)), float(1.0 - std::pow(adam->beta2, tt))};
+                           float(1.0 - std::pow(adam->beta1, tt)), float(1.0 - std::pow(adam->beta2, tt)),
+                           float(adam->weight_decay)};
+                AdamStep ab = a;
+                ab.wd = 0.f;
                 for (int l = 0; l < N->L; ++l) {
                     const int64_t nw = int64_t(N->d[l + 1]) * N->d[l], nb = N->d[l + 1];
                     k_adam<<<grid1(nw), 256, 0, N->s>>>(nw, N->W[l].p, N->dW[l].p, N->mW[l].p, N->vW[l].p, a, N->Wh[l].p);
-                    k_adam<<<grid1(nb), 256, 0, N->s>>>(nb, N->b[l].p, N->db[l].p, N->mb[l].p, N->vb[l].p, a, nullptr);
+                    k_adam<<<grid1(nb), 256, 0, N->s>>>(nb, N->b[l].p, N->db[l].p, N->mb[l].p, N->vb[l].p, ab, nullptr);
                     MCHK(hipGetLastError());
                 }
                 flops += stepFlops(*N, batch);

@@ -7,10 +7,10 @@ nonzeros per column, rows uniform in the block) plus `link` linking rows of
 planted point x0 ~ U[0, 10].
 
 Instance parameter theta in R^dim, theta ~ N(0, I).  Fixed seeded Gaussian
-maps P, Q (n x dim) turn it into
-  x0(theta) = clip(x0 * (1 + 0.1 tanh(P theta / sqrt(dim))), lo, hi)
+maps P, Q (n x dim) turn it into (defaults rhs_scale 0.1, cost_sigma 0.05)
+  x0(theta) = clip(x0 * (1 + rhs_scale tanh(P theta / sqrt(dim))), lo, hi)
   rows:  equality rows b = A x0(theta); <= rows u = A x0(theta) + slack0
-  costs: c(theta) = c0 * exp(0.05 Q theta / sqrt(dim))   (lognormal, sigma ~ 0.05)
+  costs: c(theta) = c0 * exp(cost_sigma Q theta / sqrt(dim))   (lognormal)
 so every instance is feasible (x0(theta) is inside the box and satisfies every
 row) and bounded (box-bounded columns): the right-hand sides move by up to
 +-10% through the planted point rather than by scaling b directly, which could
@@ -33,10 +33,13 @@ INF = math.inf
 
 class BlockFamily:
     def __init__(self, periods: int = 24, bm: int = 2000, bn: int = 5000, per_col: int = 5, link: int = 4000,
-                 link_nnz: int = 12, dim: int = 256, seed: int = BASE_SEED + 4):
+                 link_nnz: int = 12, dim: int = 256, seed: int = BASE_SEED + 4, rhs_scale: float = 0.1,
+                 cost_sigma: float = 0.05):
         self.base = block_lp(periods, bm, bn, per_col, link, link_nnz, seed)
         b = self.base
         self.dim = dim
+        self.rhs_scale = rhs_scale
+        self.cost_sigma = cost_sigma
         self.A = sp.csr_matrix((b.rv, b.ci, b.rp), shape=(b.m, b.n))
         g = np.random.Generator(np.random.PCG64(seed + 7919))
         self.P = g.standard_normal((b.n, dim), dtype=np.float32)
@@ -67,11 +70,11 @@ class BlockFamily:
         t32 = theta.astype(np.float32)
         px = (self.P @ t32).astype(np.float64) * s
         qx = (self.Q @ t32).astype(np.float64) * s
-        x = np.clip(b.x0 * (1.0 + 0.1 * np.tanh(px)), b.lo, b.hi)
+        x = np.clip(b.x0 * (1.0 + self.rhs_scale * np.tanh(px)), b.lo, b.hi)
         ax = self.A @ x
         rlo = np.where(self.eq, ax, -INF)
         rhi = ax + self.slack
-        c = b.c * np.exp(0.05 * qx)
+        c = b.c * np.exp(self.cost_sigma * qx)
         return c, rlo, rhi
 
     def instance(self, theta: np.ndarray, name: str = "") -> Lp:

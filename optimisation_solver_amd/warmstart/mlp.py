@@ -76,10 +76,10 @@ class Mlp:
                     "set_data")
 
     def train(self, epochs: int, batch: int = 256, lr: float = 1e-3, final_lr: float = 1e-5, beta1: float = 0.9,
-              beta2: float = 0.999, eps: float = 1e-8, seed: int = 7) -> dict:
+              beta2: float = 0.999, eps: float = 1e-8, seed: int = 7, weight_decay: float = 0.0) -> dict:
         losses = np.zeros(max(epochs, 1))
         sec, flops = C.c_double(), C.c_double()
-        a = _native.Adam(lr, beta1, beta2, eps, final_lr)
+        a = _native.Adam(lr, beta1, beta2, eps, final_lr, weight_decay)
         self._check(self.lib.pdlpx_mlp_train(self.net, epochs, batch, C.byref(a), seed, losses.ctypes.data,
                                              C.byref(sec), C.byref(flops)), "train")
         return {"epoch_loss": losses[:epochs], "seconds": sec.value, "gemm_flops": flops.value}

@@ -1,0 +1,252 @@
+"""Learned warm start, end to end on the GPU.
+
+1. Training data: GPU PDLP solves of family instances (one device batch of
+   `batch` copies of the base LP; each round swaps in new costs and row
+   bounds with pdlpx_update_vectors and solves all of them together).
+2. PCA of the primal and dual solutions: mean plus the leading `k`
+   right singular vectors of the centred solution matrices (randomised
+   subspace iteration, host BLAS), coefficients standardised per component.
+3. The network (warmstart.mlp.Mlp: BF16 MFMA kernels, Adam) maps theta to the
+   standardised primal and dual coefficients.
+4. Prediction: x = mean_x + V_x (s_x * z_x), y likewise; the engine projects a
+   warm start into the box and the dual cone before the first iteration.
+
+Cold-start PDLP is the fallback and the judge: a warm-started solve runs to
+the same tolerance with the same KKT test, and an instance whose warm solve
+does not end optimal is solved again from the cold start (its time counted).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .. import _native
+from ..solver import Engine, default_options
+from .family import BlockFamily
+from .mlp import Mlp
+
+
+def log(*a):
+    import sys
+    print(*a, file=sys.stderr, flush=True)
+
+
+@dataclass
+class SolveSet:
+    thetas: np.ndarray
+    x: np.ndarray            # count x n (float32)
+    y: np.ndarray            # count x m (float32)
+    iterations: np.ndarray
+    seconds: float
+
+
+def _load(fam: BlockFamily, batch: int, precision: int) -> Engine:
+    eng = Engine(0)
+    eng.load_batch([fam.base] * batch, precision)
+    return eng
+
+
+def _set(eng: Engine, fam: BlockFamily, thetas: np.ndarray) -> None:
+    for i, th in enumerate(thetas):
+        c, rlo, rhi = fam.vectors(th)
+        eng.update_vectors(i, c, rlo, rhi)
+
+
+def generate(fam: BlockFamily, thetas: np.ndarray, tol: float, batch: int = 64, precision: int = _native.MIXED,
+             progress_every: int = 10) -> SolveSet:
+    """Solves every theta to `tol` (count a multiple of `batch`)."""
+    count = len(thetas)
+    assert count % batch == 0, "count must be a multiple of the device batch"
+    eng = _load(fam, batch, precision)
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=precision)
+    X = np.empty((count, fam.n), np.float32)
+    Y = np.empty((count, fam.m), np.float32)
+    its = np.empty(count, np.int64)
+    t0 = time.perf_counter()
+    rounds = count // batch
+    for r in range(rounds):
+        sl = slice(r * batch, (r + 1) * batch)
+        _set(eng, fam, thetas[sl])
+        res = eng.solve_batch(o)
+        for q, s in enumerate(res):
+            if s.status != "optimal":
+                raise RuntimeError(f"training solve {r * batch + q} ended {s.status}")
+            X[r * batch + q] = s.primal
+            Y[r * batch + q] = s.row_dual
+            its[r * batch + q] = s.iterations
+        if progress_every and (r + 1) % progress_every == 0:
+            log(f"[warmstart] generated {(r + 1) * batch}/{count} solves in {time.perf_counter() - t0:.1f}s")
+    eng.close()
+    return SolveSet(thetas, X, Y, its, time.perf_counter() - t0)
+
+
+@dataclass
+class Pca:
+    mean: np.ndarray         # d
+    basis: np.ndarray        # d x k, orthonormal columns
+    scale: np.ndarray        # k: standard deviation of each coefficient
+
+    def coefficients(self, data: np.ndarray) -> np.ndarray:
+        return ((np.asarray(data, np.float64) - self.mean) @ self.basis) / self.scale
+
+    def reconstruct(self, z: np.ndarray) -> np.ndarray:
+        return self.mean + (np.asarray(z, np.float64) * self.scale) @ self.basis.T
+
+    def explained(self, data: np.ndarray) -> float:
+        """Fraction of the variance of `data` (rows) that the basis captures."""
+        d = np.asarray(data, np.float64) - self.mean
+        r = d - (d @ self.basis) @ self.basis.T
+        return float(1.0 - np.sum(r * r) / np.sum(d * d))
+
+
+def pca(data: np.ndarray, k: int, seed: int = 0, niter: int = 4, oversample: int = 16) -> tuple[Pca, np.ndarray]:
+    """Leading k principal directions of the rows of `data` and the
+    standardised coefficients of those rows.
+
+    Randomised subspace iteration (Halko, Martinsson and Tropp 2011,
+    Algorithm 4.4) on the centred matrix A (rows x features) with q = k +
+    oversample probes: Y = A W, then `niter` rounds of Y <- A (A^T Q(Y)) with
+    re-orthonormalisation, B = Q^T A and an FP64 SVD of the small q x features
+    matrix B.  Host BLAS in FP32 for the products: this runs in the same
+    process as the engine, whose ROCm runtime cannot share the GPU with a
+    second HIP runtime (torch bundles its own)."""
+    A = np.array(data, dtype=np.float32, copy=True)
+    mean = A.mean(0, dtype=np.float64)
+    A -= mean.astype(np.float32)
+    rows, feats = A.shape
+    q = min(k + oversample, rows, feats)
+    g = np.random.default_rng(seed)
+    Y = A @ g.standard_normal((feats, q), dtype=np.float32)
+    for _ in range(niter):
+        Q, _ = np.linalg.qr(Y)
+        Z, _ = np.linalg.qr(A.T @ Q)
+        Y = A @ Z
+    Q, _ = np.linalg.qr(Y.astype(np.float64))
+    B = Q.T @ A.astype(np.float64) if rows * feats <= 4e7 else (Q.T.astype(np.float32) @ A).astype(np.float64)
+    _, _, Vt = np.linalg.svd(B, full_matrices=False)
+    V = np.ascontiguousarray(Vt[:k].T)                     # features x k, orthonormal columns
+    coef = (A @ V.astype(np.float32)).astype(np.float64)  # rows x k
+    scale = np.maximum(coef.std(0), 1e-12)
+    return Pca(mean, V, scale), (coef / scale).astype(np.float32)
+
+
+class WarmStartModel:
+    """theta -> (x, y) through PCA bases and the BF16 MFMA network."""
+
+    def __init__(self, fam: BlockFamily, k_primal: int = 256, k_dual: int = 256, hidden: int = 512,
+                 seed: int = 1):
+        self.fam = fam
+        self.kp, self.kd = k_primal, k_dual
+        self.net = Mlp([fam.dim, hidden, hidden, k_primal + k_dual], max_batch=256, seed=seed)
+        self.pca_x: Pca | None = None
+        self.pca_y: Pca | None = None
+
+    def fit(self, train: SolveSet, epochs: int = 300, batch: int = 256, lr: float = 1e-3,
+            final_lr: float = 1e-5, weight_decay: float = 1e-4, rounds: int = 10, holdout: float = 0.1,
+            seed: int = 7) -> dict:
+        """PCA bases from all training solves; the network is trained on
+        (1 - holdout) of them in `rounds` stages with a decaying learning
+        rate, keeping the parameters with the lowest validation loss on the
+        held-back rest (early stopping)."""
+        t = time.perf_counter()
+        self.pca_x, zx = pca(train.x, self.kp)
+        self.pca_y, zy = pca(train.y, self.kd)
+        t_pca = time.perf_counter() - t
+        z = np.hstack([zx, zy]).astype(np.float32)
+        th = train.thetas.astype(np.float32)
+        nval = int(round(holdout * len(th))) if holdout > 0 else 0
+        ntr = (len(th) - nval) // batch * batch
+        order = np.random.default_rng(seed).permutation(len(th))
+        tri, vai = order[:ntr], order[len(th) - nval:]
+        # the last layer starts small so training starts from the mean solution
+        params = self.net.params()
+        W, b = params[-1]
+        params[-1] = (W * 0.05, b * 0.0)
+        self.net.set_params(params)
+        self.net.set_data(th[tri], z[tri])
+        per = max(1, epochs // rounds)
+        lrs = lr * (final_lr / lr) ** (np.arange(rounds + 1) / rounds)
+        best, best_params, curve, secs, flops = np.inf, self.net.params(), [], 0.0, 0.0
+        for r in range(rounds):
+            out = self.net.train(per, batch, lr=float(lrs[r]), final_lr=float(lrs[r + 1]), seed=seed + r,
+                                 weight_decay=weight_decay)
+            secs += out["seconds"]
+            flops += out["gemm_flops"]
+            val = float(np.mean((self.net.forward(th[vai]) - z[vai]) ** 2)) if nval else out["epoch_loss"][-1]
+            curve.append((float(out["epoch_loss"][-1]), val))
+            if val < best:
+                best, best_params = val, self.net.params()
+        self.net.set_params(best_params)
+        sub = slice(0, min(512, len(train.x)))
+        return {"train_loss": [c[0] for c in curve], "val_loss": [c[1] for c in curve], "best_val_loss": best,
+                "seconds": secs, "gemm_flops": flops, "pca_seconds": t_pca,
+                "explained_x": self.pca_x.explained(train.x[sub]), "explained_y": self.pca_y.explained(train.y[sub])}
+
+    def predict(self, thetas: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+        z = self.net.forward(np.asarray(thetas, np.float32)).astype(np.float64)
+        x = self.pca_x.reconstruct(z[:, :self.kp])
+        y = self.pca_y.reconstruct(z[:, self.kp:])
+        return x, y
+
+    def close(self) -> None:
+        self.net.close()
+
+
+@dataclass
+class Comparison:
+    cold_iterations: np.ndarray
+    warm_iterations: np.ndarray
+    cold_seconds: np.ndarray     # per-instance solve time (its batch's run)
+    warm_seconds: np.ndarray
+    cold_wall: float             # wall time of all cold batches
+    warm_wall: float
+    fallbacks: int
+    cold_kkt: np.ndarray
+    warm_kkt: np.ndarray
+
+
+def compare(fam: BlockFamily, thetas: np.ndarray, warm_x, warm_y, tol: float = 1e-6, batch: int = 64,
+            cold_precision: int = _native.FP64, warm_precision: int = _native.FP64) -> Comparison:
+    """Cold and warm-started solves of the same held-out instances, batch by batch."""
+    count = len(thetas)
+    assert count % batch == 0
+    ce = _load(fam, batch, cold_precision)
+    we = ce if warm_precision == cold_precision else _load(fam, batch, warm_precision)
+    oc = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=cold_precision)
+    ow = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=warm_precision)
+    ci, wi = np.zeros(count, np.int64), np.zeros(count, np.int64)
+    cs, ws = np.zeros(count), np.zeros(count)
+    ck, wk = np.zeros(count), np.zeros(count)
+    cw = ww = 0.0
+    fallbacks = 0
+    for r in range(count // batch):
+        sl = slice(r * batch, (r + 1) * batch)
+        _set(ce, fam, thetas[sl])
+        t = time.perf_counter()
+        cold = ce.solve_batch(oc)
+        cw += time.perf_counter() - t
+        if we is not ce:
+            _set(we, fam, thetas[sl])
+        t = time.perf_counter()
+        warm = we.solve_batch(ow, list(warm_x[sl]), list(warm_y[sl]))
+        ww += time.perf_counter() - t
+        for q in range(batch):
+            i = r * batch + q
+            c, w = cold[q], warm[q]
+            ci[i], cs[i] = c.iterations, c.solve_time_seconds
+            ck[i] = max(c.primal_residual, c.dual_residual, c.relative_gap)
+            wi[i], ws[i] = w.iterations, w.solve_time_seconds
+            wk[i] = max(w.primal_residual, w.dual_residual, w.relative_gap)
+            if w.status != "optimal":   # cold start is the fallback
+                fallbacks += 1
+                wi[i] += c.iterations
+                ws[i] += c.solve_time_seconds
+                wk[i] = ck[i]
+        log(f"[warmstart] held-out {(r + 1) * batch}/{count}: cold its p50 {np.median(ci[:i + 1]):.0f}, "
+            f"warm its p50 {np.median(wi[:i + 1]):.0f}")
+    ce.close()
+    if we is not ce:
+        we.close()
+    return Comparison(ci, wi, cs, ws, cw, ww, fallbacks, ck, wk)
