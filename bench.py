@@ -211,6 +211,13 @@ def bench_warmstart(args) -> int:
     wx, wy = model.predict(th)
     mean_x = np.repeat(model.pca_x.mean[None], nte, 0)
     mean_y = np.repeat(model.pca_y.mean[None], nte, 0)
+    px, py = model.pca_x, model.pca_y
+
+    def model_px(v):
+        return px.reconstruct(px.coefficients(v))
+
+    def model_py(v):
+        return py.reconstruct(py.coefficients(v))
     model.close()
     res = {}
     for tag, xs, ys, prec in (("warm_fp64", wx, wy, _native.FP64), ("warm_mixed", wx, wy, _native.MIXED),
@@ -218,6 +225,21 @@ def bench_warmstart(args) -> int:
         log(f"[bench] held-out: cold FP64 vs {tag}")
         res[tag] = P.compare(fam, th, xs, ys, tol=1e-6, batch=B, cold_precision=_native.FP64, warm_precision=prec)
     cold = res["warm_fp64"]
+    ablation = None
+    if args.ablation:
+        log("[bench] ablation: cold solutions, then warm starts from perturbed truth and its PCA projection")
+        truth = P.solve_set(fam, th, tol=1e-6, batch=B, keep=True)
+        g = np.random.default_rng(1)
+        ablation = {"cold_iterations_p50": float(np.median(truth["iterations"]))}
+        starts = {"truth_noise_1e-2": (truth["x"] * (1 + 1e-2 * g.standard_normal(truth["x"].shape)),
+                                       truth["y"] * (1 + 1e-2 * g.standard_normal(truth["y"].shape))),
+                  "truth_noise_1e-3": (truth["x"] * (1 + 1e-3 * g.standard_normal(truth["x"].shape)),
+                                       truth["y"] * (1 + 1e-3 * g.standard_normal(truth["y"].shape))),
+                  "pca_projection_of_truth": (model_px(truth["x"]), model_py(truth["y"]))}
+        for tag, (sx, sy) in starts.items():
+            w = P.solve_set(fam, th, sx, sy, tol=1e-6, batch=B)
+            ablation[tag] = round(float(np.median(truth["iterations"] / np.maximum(w["iterations"], 1))), 3)
+            log(f"[bench] ablation {tag}: median iteration ratio {ablation[tag]}")
 
     def summary(c):
         return {"iterations_p50": float(np.median(c.warm_iterations)),
@@ -243,7 +265,8 @@ def bench_warmstart(args) -> int:
                    "network": {"dims": [fam.dim, 512, 512, 512], "best_val_loss": fit["best_val_loss"],
                                "val_loss_by_round": fit["val_loss"], "explained_x": fit["explained_x"],
                                "explained_y": fit["explained_y"], "train_s": round(fit["seconds"], 3),
-                               "pca_s": round(fit["pca_seconds"], 2)}},
+                               "pca_s": round(fit["pca_seconds"], 2)},
+                   "ablation_median_iteration_ratio": ablation},
         "roofline": {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(mlp_tflops / 2500.0, 5), "traffic": None, "kernel": "k_gemm (bf16 MFMA)",
                      "note": "network training GEMMs (batch 256, 256-512 wide): latency-bound by size"},
@@ -265,6 +288,9 @@ def main() -> int:
     ap.add_argument("--test-count", type=int, default=256, help="held-out instances of the warm-start workload")
     ap.add_argument("--train-tolerance", type=float, default=1e-4)
     ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--ablation", action="store_true",
+                    help="warm-start workload: also start from the true solution with 1%% and 0.1%% noise and "
+                         "from its exact PCA projection (what accuracy PDLP needs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default=None,
                     help="PDHG arithmetic: fp64 (reference), fp32 (FP32 values and iterates), mixed (FP32 "
@@ -313,7 +339,10 @@ def main() -> int:
 
     eng = Engine(0)
     eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... iteration {it}, best KKT {kkt:.3e}"), 20.0)
+    t = time.perf_counter()
     eng.load(lp, prec)
+    load_s = time.perf_counter() - t
+    log(f"[bench] load (validation, degree sort, row-block plan, host-to-device copy): {load_s:.3f}s")
     tiling = None
     if args.autotune:
         t = time.perf_counter()
@@ -377,6 +406,7 @@ def main() -> int:
                    "fp32_iterations": res.diagnostics["fp32_iterations"],
                    "device_loop_s": round(res.diagnostics["device_loop_seconds"], 4),
                    "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name,
+                   "load_s": round(load_s, 4),
                    "tiling": tiling or eng_tiling},
         "roofline": roof, "cpu_baseline": cpu, "parity": parity,
         "capped": args.iteration_limit < 10_000_000,

@@ -207,6 +207,37 @@ class Comparison:
     warm_kkt: np.ndarray
 
 
+def solve_set(fam: BlockFamily, thetas: np.ndarray, warm_x=None, warm_y=None, tol: float = 1e-6,
+              batch: int = 64, precision: int = _native.FP64, keep: bool = False) -> dict:
+    """Solves the instances (warm-started where warm_x/warm_y are given);
+    per-instance iterations, seconds, status and KKT; solutions if `keep`."""
+    count = len(thetas)
+    assert count % batch == 0
+    eng = _load(fam, batch, precision)
+    o = default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, precision=precision)
+    out = {"iterations": np.zeros(count, np.int64), "seconds": np.zeros(count), "kkt": np.zeros(count),
+           "optimal": np.zeros(count, bool), "wall": 0.0}
+    if keep:
+        out["x"] = np.empty((count, fam.n))
+        out["y"] = np.empty((count, fam.m))
+    for r in range(count // batch):
+        sl = slice(r * batch, (r + 1) * batch)
+        _set(eng, fam, thetas[sl])
+        t = time.perf_counter()
+        res = eng.solve_batch(o, None if warm_x is None else list(warm_x[sl]),
+                              None if warm_y is None else list(warm_y[sl]))
+        out["wall"] += time.perf_counter() - t
+        for q, s in enumerate(res):
+            i = r * batch + q
+            out["iterations"][i], out["seconds"][i] = s.iterations, s.solve_time_seconds
+            out["kkt"][i] = max(s.primal_residual, s.dual_residual, s.relative_gap)
+            out["optimal"][i] = s.status == "optimal"
+            if keep:
+                out["x"][i], out["y"][i] = s.primal, s.row_dual
+    eng.close()
+    return out
+
+
 def compare(fam: BlockFamily, thetas: np.ndarray, warm_x, warm_y, tol: float = 1e-6, batch: int = 64,
             cold_precision: int = _native.FP64, warm_precision: int = _native.FP64) -> Comparison:
     """Cold and warm-started solves of the same held-out instances, batch by batch."""
