@@ -221,11 +221,16 @@ int pdlpx_scaling(pdlpx_ctx* ctx, int32_t ruiz_iterations, int32_t use_precondit
                   double* primal_scale, double* dual_scale);
 
 /*
- * Times the two PDHG trial kernels on the state the last pdlpx_solve left on
- * the device: `reps` back-to-back launches of the primal half (A^T y + prox)
- * and of the dual half (A x' + prox), each bracketed by hipEvents on the
- * context's stream.  Writes the average milliseconds per launch.  A profiling
- * aid with no reference counterpart; the iterate is not modified.
+ * Times the two halves of a PDHG trial as the solve issues them: on the state
+ * the last solve left on the device, `reps` trials of (primal half, dual half,
+ * linesearch decision) run in sequence on the context's stream, with hipEvents
+ * between the kernels; writes the average milliseconds of the primal half
+ * (A^T y + prox, both kernels when row-sliced) and of the dual half (A x' +
+ * prox).  Each half therefore starts with the caches the previous one left, as
+ * in the solve.  A profiling aid with no reference counterpart: the trials
+ * advance the device iterate (the per-instance control state is restored
+ * afterwards); solutions already returned are unaffected, and the next solve
+ * starts afresh as always.
  */
 int pdlpx_time_trial(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
                      double* ms_primal, double* ms_dual);
@@ -251,6 +256,21 @@ int pdlpx_set_progress(pdlpx_ctx* ctx, pdlpx_progress_fn fn, void* user, double 
  */
 int pdlpx_update_vectors(pdlpx_ctx* ctx, int32_t index, const double* objective,
                          const double* row_lower, const double* row_upper);
+
+/*
+ * Row slicing of the primal half.  When an instance's y is larger than a
+ * fraction of one XCD's L2, A^T y is computed over a reordered copy of A^T
+ * whose rows are (slice of A's rows, column) pairs taken slice by slice, so
+ * every block in flight gathers from one L2-resident slice of y; a second
+ * pass sums each column's slice partials in slice order and applies the
+ * primal prox.  pdlpx_load enables it with 131072 rows per slice when some
+ * instance has more than 524288 rows.  pdlpx_set_slicing rebuilds it with
+ * `rows_per_slice` rows per slice (0 = off; an instance with fewer rows gets
+ * one slice).  Results equal the unsliced ones up to summation order.
+ */
+int pdlpx_set_slicing(pdlpx_ctx* ctx, int32_t rows_per_slice);
+/* rows_per_slice (0 when off) and the number of (slice, column) rows. */
+int pdlpx_get_slicing(pdlpx_ctx* ctx, int32_t* rows_per_slice, int32_t* slice_rows_total);
 
 /*
  * SpMV tiling of the loaded problem.  Each matrix orientation (CSR rows of A

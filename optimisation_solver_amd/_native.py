@@ -135,6 +135,8 @@ def engine() -> C.CDLL:
         lib.pdlpx_mlp_set_data.argtypes = [vp, i32, vp, vp]
         lib.pdlpx_mlp_train.argtypes = [vp, i32, i32, C.POINTER(Adam), C.c_uint64, vp, pf64, pf64]
         lib.pdlpx_mlp_gemm.argtypes = [vp, i32, i32, i32, vp, i32, vp, i32, vp, i32, pf64]
+        lib.pdlpx_set_slicing.argtypes = [vp, i32]
+        lib.pdlpx_get_slicing.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
         lib.pdlpx_update_vectors.argtypes = [vp, i32, vp, vp, vp]
         lib.pdlpx_autotune.argtypes = [vp, i32, i32, C.POINTER(Tiling)]
         lib.pdlpx_set_tiling.argtypes = [vp, C.POINTER(Tiling)]

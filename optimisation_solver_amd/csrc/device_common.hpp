@@ -247,6 +247,41 @@ __device__ inline void row_block(const Orient<V>& M, const int4 bm, const T* __r
     if (mine && g == 0) epi.apply(r, s, pre);
 }
 
+// Row block of up to kTileRowsMany rows whose epilogue needs no per-row data
+// (sums stored by apply(row, sum)): several rows per thread when the block
+// has more rows than threads, so blocks of short rows still fill a tile of
+// nonzeros.  Rows longer than a tile are not allowed here (the plan caps the
+// nonzeros of every block at kTileNnz).
+constexpr int kTileRowsMany = 4 * kThreads;
+
+template <typename V, typename T, typename Epi>
+__device__ inline void row_block_many(const Orient<V>& M, const int4 bm, const T* __restrict__ v, T* lds,
+                                      Epi& epi) {
+    const int r0 = bm.x, r1 = bm.y, k0 = bm.z, k1 = bm.w;
+    const int t = threadIdx.x;
+    const int rows = r1 - r0;
+    int a[kTileRowsMany / kThreads], e[kTileRowsMany / kThreads];
+#pragma unroll
+    for (int u = 0; u < kTileRowsMany / kThreads; ++u) {
+        const int r = r0 + t + u * kThreads;
+        a[u] = e[u] = 0;
+        if (t + u * kThreads < rows) {
+            a[u] = M.ptr[r];
+            e[u] = M.ptr[r + 1];
+        }
+    }
+    stage_chunk(M, v, k0, k1, lds);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kTileRowsMany / kThreads; ++u) {
+        if (t + u * kThreads < rows) {
+            T s = T(0);
+            for (int q = a[u] - k0; q < e[u] - k0; ++q) s += lds[q];
+            epi.apply(r0 + t + u * kThreads, s);
+        }
+    }
+}
+
 // Two vectors through the same block (termination checks score the current
 // and the averaged iterate in one matrix pass).  `lds` holds 2 * kTileNnz.
 // The epilogue's apply takes (row, sum0, sum1, pre).

@@ -114,7 +114,8 @@ void partitionRows(const std::vector<int32_t>& ptr, const std::vector<int32_t>& 
 
 // Streaming segments of at most kSegLen elements inside each instance range;
 // ibeg[k] = first segment of instance k (counted from the first segment added).
-void makeSegments(const std::vector<int32_t>& instStart, std::vector<Seg>& segs, std::vector<int32_t>& ibeg) {
+void makeSegments(const std::vector<int32_t>& instStart, std::vector<Seg>& segs, std::vector<int32_t>& ibeg,
+                  int segLen = kSegLen) {
     const int B = static_cast<int>(instStart.size()) - 1;
     const int32_t first = static_cast<int32_t>(segs.size());
     ibeg.assign(1, 0);
@@ -123,7 +124,7 @@ void makeSegments(const std::vector<int32_t>& instStart, std::vector<Seg>& segs,
         const int e = instStart[k + 1];
         int made = 0;
         while (a < e) {
-            const int b = std::min(e, a + kSegLen);
+            const int b = std::min(e, a + segLen);
             segs.push_back(Seg{k, a, b, 0});
             a = b;
             ++made;
@@ -235,6 +236,18 @@ struct pdlpx_ctx {
     DBuf<Seg> segs;
     int ncs = 0, nrs = 0;
     DBuf<int32_t> colSegBeg, rowSegBeg, rowSegBegAll;
+    // Row-sliced copy of A^T for the primal half (used when `sliced`): rows
+    // are (slice of A's rows, column) pairs, slice-major inside each instance.
+    bool sliced = false;
+    int32_t sliceRows = 0;             // rows of A per slice
+    OrientBuf SAT;
+    DBuf<int32_t> satPerm;             // SAT nonzero -> A^T nonzero
+    DBuf<int32_t> cptr, cidx;          // column j -> its SAT rows, in slice order
+    DBuf<double> satV64, partS64;
+    DBuf<float> satV32, partS32;
+    DBuf<Seg> csegs;                   // column segments of kThreads columns (the combine pass)
+    DBuf<int32_t> cSegBeg;
+    int nccs = 0;
 
     // ---- per-solve work ----
     DBuf<double> rs, cs, rmax, cmax;
@@ -307,6 +320,35 @@ bool optionsValid(const pdlpx_options& o) {   // pdlp_solver.cpp:127-133
              o.dual_tolerance < 0.0 || o.gap_tolerance < 0.0 || o.initial_step_safety <= 0.0 ||
              o.minimum_primal_weight <= 0.0 || o.maximum_primal_weight < o.minimum_primal_weight);
 }
+
+template <typename V>
+const V* satValues(const pdlpx_ctx& X) {
+    if constexpr (sizeof(V) == 4) return X.satV32.p;
+    else return X.satV64.p;
+}
+template <typename T>
+T* satParts(pdlpx_ctx& X) {
+    if constexpr (sizeof(T) == 4) return X.partS32.p;
+    else return X.partS64.p;
+}
+
+// The primal half of a trial for the instances in phase (V, T): one fused
+// row-block kernel over A^T, or the row-sliced pair when the context is sliced.
+template <typename V, typename T>
+void launchPrimalHalf(pdlpx_ctx& X, hipStream_t s, const V* colVals, ColData<T> cols, Iterates<T> it) {
+    if (X.sliced) {
+        k_primal_slices<V, T><<<X.SAT.nblk, kThreads, 0, s>>>(X.SAT.view<V>(satValues<V>(X)), it, X.st.p,
+                                                              satParts<T>(X));
+        k_primal_combine<V, T><<<X.nccs, kThreads, 0, s>>>(X.csegs.p, cols, it, X.st.p, X.cptr.p, X.cidx.p,
+                                                           satParts<T>(X), X.part_p.p);
+    } else {
+        k_primal<V, T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<V>(colVals), cols, it, X.st.p, X.part_p.p);
+    }
+}
+
+// Per-instance first partial of the primal half (row blocks of A^T, or
+// column segments when sliced).
+const int32_t* primalBegins(const pdlpx_ctx& X) { return X.sliced ? X.cSegBeg.p : X.AT.ibeg.p; }
 
 // ===========================================================================
 // One batched solve.
@@ -432,8 +474,7 @@ private:
     }
     template <typename V, typename T>
     void launchPrimal() {
-        k_primal<V, T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<V>(colValues<V>()), work<T>().cols(),
-                                                      iters<T>().view(), X.st.p, X.part_p.p);
+        launchPrimalHalf<V, T>(X, s, colValues<V>(), work<T>().cols(), iters<T>().view());
     }
     template <typename V, typename T>
     void launchDual() {
@@ -451,9 +492,16 @@ private:
         if (phases & 1) launchDual<double, double>();
         if (phases & 2) launchDual<float, float>();
         if (phases & 4) launchDual<float, double>();
-        k_decide<<<B, kReduceThreads, 0, s>>>(X.st.p, X.AT.ibeg.p, X.A.ibeg.p, X.part_p.p, X.part_d.p, X.part_i.p);
+        k_decide<<<B, kReduceThreads, 0, s>>>(X.st.p, primalBegins(X), X.A.ibeg.p, X.part_p.p, X.part_d.p,
+                                              X.part_i.p);
+    }
+    // Applies the averaging update still pending after the last trial of a
+    // launch sequence, then clears the flags.
+    void flushAverage(int phases) {
         if (phases & 5) launchAverage<double>();
         if (phases & 2) launchAverage<float>();
+        k_clear_avg<<<gridFor(B), kThreads, 0, s>>>(X.st.p, B);
+        HIPCHK(hipGetLastError());
     }
     hipGraphExec_t graphFor(int phases, int slots) {
         for (auto& g : X.graphs)
@@ -493,6 +541,7 @@ private:
             } else {
                 HIPCHK(hipGraphLaunch(graphFor(phases, slots), s));
             }
+            flushAverage(phases);
             pullState();
             bool any = false;
             left = 0;
@@ -776,6 +825,11 @@ void Run::setup() {
     k_scale_values<<<gridFor(m, bt), bt, 0, s>>>(m, X.A.ptr.p, X.A.idx.p, X.A.val.p, X.rs.p, X.cs.p, w.vr.p, v32r);
     k_scale_values<<<gridFor(n, bt), bt, 0, s>>>(n, X.AT.ptr.p, X.AT.idx.p, X.AT.val.p, X.cs.p, X.rs.p, w.vc.p,
                                                  v32c);
+    if (X.sliced && X.nnz > 0) {
+        const int g = std::min<int64_t>(8192, gridFor(X.nnz));
+        k_permute<double><<<g, kThreads, 0, s>>>(X.nnz, X.satPerm.p, w.vc.p, X.satV64.p);
+        if (f32) k_permute<float><<<g, kThreads, 0, s>>>(X.nnz, X.satPerm.p, v32c, X.satV32.p);
+    }
     k_scale_cols<<<gridFor(n, bt), bt, 0, s>>>(n, X.cs.p, X.c.p, X.lo.p, X.hi.p, w.c.p, w.lo.p, w.hi.p);
     k_scale_rows<<<gridFor(m, bt), bt, 0, s>>>(m, X.rs.p, X.rlo.p, X.rhi.p, w.rlo.p, w.rhi.p);
     if (o.use_diagonal_preconditioning) {
@@ -1381,9 +1435,114 @@ void uploadOrient(OrientBuf& ob, const std::vector<int32_t>& ptr, const std::vec
     HIPCHK(hipStreamSynchronize(s));   // the local staging vectors die here
 }
 
+// Rows of A per slice for the row-sliced primal half: slicing pays when an
+// instance's y (FP64) is larger than about half of one XCD's 4 MB L2.
+int32_t autoSliceRows(const pdlpx_ctx& X) {
+    int maxM = 0;
+    for (int k = 0; k < X.B; ++k) maxM = std::max(maxM, X.rowOff[k + 1] - X.rowOff[k]);
+    return int64_t(maxM) * 8 > (int64_t(4) << 20) ? 131072 : 0;
+}
+
+// Builds the row-sliced copy of A^T from its CSR (ptr = column starts,
+// idx = internal row indices): instance k's rows are cut into
+// ceil(m_k / sliceRows) equal ranges; its nonzeros are stably sorted by slice
+// (column order and entry order kept inside a slice), consecutive entries of
+// one column form one SAT row.
+void buildSlices(pdlpx_ctx& X, const std::vector<int32_t>& cp, const std::vector<int32_t>& ri, hipStream_t s) {
+    X.sliced = X.sliceRows > 0 && X.B > 0;
+    if (!X.sliced) {
+        X.SAT.nblk = X.SAT.nrows = 0;
+        for (auto* d : {&X.SAT.ptr, &X.SAT.idx, &X.SAT.binst, &X.SAT.ibeg, &X.satPerm, &X.cptr, &X.cidx})
+            d->release();
+        X.SAT.meta.release();
+        for (auto* d : {&X.satV64, &X.partS64}) d->release();
+        for (auto* d : {&X.satV32, &X.partS32}) d->release();
+        return;
+    }
+    const int64_t Z = X.nnz;
+    std::vector<int32_t> sid(Z), colOf(Z), perm(Z), sptr, rowCol, instStart(X.B + 1, 0);
+    for (int j = 0; j < X.n; ++j)
+        for (int32_t e = cp[j]; e < cp[j + 1]; ++e) colOf[e] = j;
+    std::vector<int64_t> cnt;
+    for (int k = 0; k < X.B; ++k) {
+        const int R = X.rowOff[k], mk = X.rowOff[k + 1] - R;
+        const int S = std::max(1, (mk + X.sliceRows - 1) / X.sliceRows);
+        const int32_t e0 = cp[X.colOff[k]], e1 = cp[X.colOff[k + 1]];
+        cnt.assign(size_t(S) + 1, 0);
+        for (int32_t e = e0; e < e1; ++e) {
+            sid[e] = static_cast<int32_t>(int64_t(ri[e] - R) * S / mk);
+            ++cnt[sid[e] + 1];
+        }
+        for (int q = 0; q < S; ++q) cnt[q + 1] += cnt[q];
+        for (int32_t e = e0; e < e1; ++e) perm[e0 + cnt[sid[e]]++] = e;
+        instStart[k] = static_cast<int32_t>(rowCol.size());
+        for (int32_t q = e0; q < e1; ++q) {
+            const int32_t e = perm[q];
+            if (q == e0 || sid[e] != sid[perm[q - 1]] || colOf[e] != colOf[perm[q - 1]]) {
+                sptr.push_back(q);
+                rowCol.push_back(colOf[e]);
+            }
+        }
+    }
+    const int32_t R = static_cast<int32_t>(rowCol.size());
+    instStart[X.B] = R;
+    sptr.push_back(static_cast<int32_t>(Z));
+    // A (slice, column) row longer than a tile is split into pieces of at
+    // most kTileNnz entries (consecutive SAT rows of the same column), so
+    // every block fits the tile of k_primal_slices.
+    {
+        std::vector<int32_t> sp2, rc2;
+        sp2.reserve(sptr.size());
+        rc2.reserve(rowCol.size());
+        std::vector<int32_t> inst2(X.B + 1, 0);
+        int k = 0;
+        for (int32_t r = 0; r < R; ++r) {
+            while (k < X.B && instStart[k + 1] <= r) inst2[++k] = static_cast<int32_t>(rc2.size());
+            for (int32_t a = sptr[r]; a < sptr[r + 1]; a += kTileNnz) {
+                sp2.push_back(a);
+                rc2.push_back(rowCol[r]);
+            }
+        }
+        while (k < X.B) inst2[++k] = static_cast<int32_t>(rc2.size());
+        sptr.swap(sp2);
+        rowCol.swap(rc2);
+        instStart.swap(inst2);
+    }
+    const int32_t R2 = static_cast<int32_t>(rowCol.size());
+    sptr.push_back(static_cast<int32_t>(Z));
+    std::vector<int32_t> sidx(Z), cptr(size_t(X.n) + 1, 0), cidx(R2);
+    for (int64_t q = 0; q < Z; ++q) sidx[q] = ri[perm[q]];
+    for (int32_t r = 0; r < R2; ++r) ++cptr[rowCol[r] + 1];
+    for (int j = 0; j < X.n; ++j) cptr[j + 1] += cptr[j];
+    std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+    for (int32_t r = 0; r < R2; ++r) cidx[fill[rowCol[r]]++] = r;
+    OrientBuf& M = X.SAT;
+    M.nrows = R2;
+    M.hptr = sptr;
+    M.plan(instStart, kTileNnz, kTileRowsMany, s);
+    M.ptr.upload(sptr.data(), sptr.size(), s);
+    M.idx.upload(sidx.data(), sidx.size(), s);
+    X.satPerm.upload(perm.data(), perm.size(), s);
+    X.cptr.upload(cptr.data(), cptr.size(), s);
+    X.cidx.upload(cidx.data(), cidx.size(), s);
+    X.satV64.alloc(size_t(Z));
+    X.partS64.alloc(size_t(R2));
+    std::vector<Seg> cs;
+    std::vector<int32_t> cbeg;
+    makeSegments(X.colOff, cs, cbeg, kThreads);
+    X.nccs = static_cast<int>(cs.size());
+    X.csegs.upload(cs.data(), cs.size(), s);
+    X.cSegBeg.upload(cbeg.data(), cbeg.size(), s);
+    if (X.have32) {
+        X.satV32.alloc(size_t(Z));
+        X.partS32.alloc(size_t(R2));
+    }
+    HIPCHK(hipStreamSynchronize(s));   // the staging vectors die here
+}
+
 // Per-block partial sums of every row-block kernel (sized by the plans).
 void allocPartials(pdlpx_ctx& X) {
-    X.part_p.alloc(X.AT.nblk);
+    X.part_p.alloc(std::max(X.AT.nblk, X.nccs));
     X.part_d.alloc(X.A.nblk);
     X.part_i.alloc(X.A.nblk);
     X.part_rows.alloc(6 * size_t(X.A.nblk));
@@ -1416,6 +1575,10 @@ void allocWork(pdlpx_ctx& X, bool fp32) {
         for (auto* b : {&h.c, &h.lo, &h.hi, &h.tau}) b->alloc(n);
         for (auto* b : {&h.rlo, &h.rhi, &h.sig}) b->alloc(m);
         X.it32.alloc(n, m);
+        if (X.sliced) {
+            X.satV32.alloc(nz);
+            X.partS32.alloc(size_t(X.SAT.nrows));
+        }
         X.have32 = true;
     }
 }
@@ -1546,6 +1709,9 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     X.colSegBeg.upload(cbeg.data(), cbeg.size(), s);
     X.rowSegBeg.upload(rbeg.data(), rbeg.size(), s);
     X.rowSegBegAll.upload(rall.data(), rall.size(), s);
+    X.have32 = false;
+    X.sliceRows = autoSliceRows(X);
+    buildSlices(X, cp, ri, s);
     allocWork(X, precision != PDLPX_FP64);
     HIPCHK(hipStreamSynchronize(s));   // host staging vectors die here
 }
@@ -1761,6 +1927,29 @@ int pdlpx_spmv_device(pdlpx_ctx* X, int32_t transpose, const void* x, void* y) {
                                                                     static_cast<const double*>(x),
                                                                     static_cast<double*>(y));
         HIPCHK(hipGetLastError());
+    });
+}
+
+int pdlpx_set_slicing(pdlpx_ctx* X, int32_t rows_per_slice) {
+    if (rows_per_slice < 0) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        hipStream_t s = X->stream;
+        std::vector<int32_t> ri(static_cast<size_t>(X->nnz));
+        if (X->nnz) HIPCHK(hipMemcpyAsync(ri.data(), X->AT.idx.p, sizeof(int32_t) * ri.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        X->dropGraphs();
+        X->sliceRows = rows_per_slice;
+        buildSlices(*X, X->AT.hptr, ri, s);
+        allocPartials(*X);
+    });
+}
+
+int pdlpx_get_slicing(pdlpx_ctx* X, int32_t* rows_per_slice, int32_t* slice_rows_total) {
+    return guarded(X, [&] {
+        if (!X->loaded) throw Failure{PDLPX_E_STATE, "no problem loaded"};
+        if (rows_per_slice) *rows_per_slice = X->sliced ? X->sliceRows : 0;
+        if (slice_rows_total) *slice_rows_total = X->sliced ? X->SAT.nrows : 0;
     });
 }
 
@@ -2000,22 +2189,16 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
         HIPCHK(hipMemcpyAsync(saved.data(), X->st.p, sizeof(DevState) * B, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
         probe = saved;
-        for (auto& h : probe) {
+        for (auto& h : probe) {   // every instance running, in the requested phase, with no target
             h.done = 0;
+            h.fail = 0;
             h.phase = precision;
+            h.target = std::numeric_limits<int64_t>::max();
         }
         HIPCHK(hipMemcpyAsync(X->st.p, probe.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
-        auto timeIt = [&](auto&& launch) {
-            launch();   // warm
-            HIPCHK(hipEventRecord(X->ev0, s));
-            for (int r = 0; r < reps; ++r) launch();
-            HIPCHK(hipEventRecord(X->ev1, s));
-            HIPCHK(hipEventSynchronize(X->ev1));
-            float ms = 0.0f;
-            HIPCHK(hipEventElapsedTime(&ms, X->ev0, X->ev1));
-            return static_cast<double>(ms) / reps;
-        };
-        auto both = [&](auto vtag, auto ttag) {
+        std::vector<hipEvent_t> ev(size_t(2) * reps + 1, nullptr);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        auto seq = [&](auto vtag, auto ttag) {
             using V = decltype(vtag);
             using T = decltype(ttag);
             const V* vc;
@@ -2036,19 +2219,43 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
                 wp = &X->w64;
                 ip = &X->it64;
             }
-            *ms_primal = timeIt([&] {
-                k_primal<V, T><<<X->AT.nblk, kThreads, 0, s>>>(X->AT.view<V>(vc), wp->cols(), ip->view(), X->st.p,
-                                                               X->part_p.p);
-            });
-            *ms_dual = timeIt([&] {
+            auto dual = [&] {
                 k_dual<V, T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<V>(vr), wp->rows(), ip->view(), X->st.p,
                                                             X->part_d.p, X->part_i.p);
-            });
+            };
+            auto decide = [&] {
+                k_decide<<<B, kReduceThreads, 0, s>>>(X->st.p, primalBegins(*X), X->A.ibeg.p, X->part_p.p,
+                                                      X->part_d.p, X->part_i.p);
+            };
+            // one warm trial, then `reps` trials exactly as the solve issues them
+            launchPrimalHalf<V, T>(*X, s, vc, wp->cols(), ip->view());
+            dual();
+            decide();
+            for (int r = 0; r < reps; ++r) {
+                HIPCHK(hipEventRecord(ev[2 * r], s));
+                launchPrimalHalf<V, T>(*X, s, vc, wp->cols(), ip->view());
+                HIPCHK(hipEventRecord(ev[2 * r + 1], s));
+                dual();
+                HIPCHK(hipEventRecord(ev[2 * r + 2], s));
+                decide();
+            }
         };
-        if (precision == PDLPX_FP32) both(float{}, float{});
-        else if (precision == PDLPX_MIXED) both(float{}, double{});
-        else both(double{}, double{});
+        if (precision == PDLPX_FP32) seq(float{}, float{});
+        else if (precision == PDLPX_MIXED) seq(float{}, double{});
+        else seq(double{}, double{});
         HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        double tp = 0.0, td = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            float a = 0.f, b = 0.f;
+            HIPCHK(hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]));
+            tp += a;
+            td += b;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *ms_primal = tp / reps;
+        *ms_dual = td / reps;
         HIPCHK(hipMemcpyAsync(X->st.p, saved.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
         HIPCHK(hipStreamSynchronize(s));
     });

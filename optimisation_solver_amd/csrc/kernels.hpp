@@ -126,11 +126,18 @@ struct PrimalEpi {
     T* __restrict__ xt;
     T mul;
     double acc = 0.0;
+    T* __restrict__ avg = nullptr;   // running average, updated when `upd`
+    T af = T(0);
+    int upd = 0;
     __device__ Pre load(int j) const {
         return Pre{stream_load(cd.c + j), stream_load(cd.lo + j), stream_load(cd.hi + j), stream_load(cd.tau + j),
                    stream_load(x + j)};
     }
     __device__ void apply(int j, T s, const Pre& p) {
+        if (upd) {   // the previous trial's accepted iterate into the average
+            const T a = avg[j];
+            avg[j] = a + af * (p.x - a);
+        }
         const T g = p.c + s;
         const T now = clip(p.x - mul * p.tau * g, p.lo, p.hi);
         xt[j] = now;
@@ -154,6 +161,9 @@ __global__ __launch_bounds__(kThreads) void k_primal(Orient<V> at, ColData<T> cd
     __shared__ double sh[kWaves];
     const int p = st.parity;
     PrimalEpi<T> epi{cd, it.x[p], it.x[p ^ 1], static_cast<T>(st.eta_trial / st.omega)};
+    epi.avg = it.avg_x;
+    epi.af = static_cast<T>(st.avg_frac);
+    epi.upd = st.avg_pending;
     row_block(at, bm, it.y[p], lds, shT, epi);
     double v[1] = {epi.acc};
     block_sums<1>(v, sh);
@@ -172,11 +182,18 @@ struct DualEpi {
     T* __restrict__ axt;
     T mul;
     double mov = 0.0, inter = 0.0;
+    T* __restrict__ avg = nullptr;
+    T af = T(0);
+    int upd = 0;
     __device__ Pre load(int i) const {
         return Pre{stream_load(rd.lo + i), stream_load(rd.hi + i), stream_load(rd.sig + i), stream_load(y + i),
                    stream_load(ax + i)};
     }
     __device__ void apply(int i, T act, const Pre& p) {
+        if (upd) {
+            const T a = avg[i];
+            avg[i] = a + af * (p.y - a);
+        }
         const T dact = act - p.ax;
         const T ext = act + dact;
         const T step = smax(mul * p.sig, floor_step<T>());
@@ -206,6 +223,9 @@ __global__ __launch_bounds__(kThreads) void k_dual(Orient<V> a, RowData<T> rd, I
     __shared__ double sh[2 * kWaves];
     const int p = st.parity;
     DualEpi<T> epi{rd, it.y[p], it.ax[p], it.y[p ^ 1], it.ax[p ^ 1], static_cast<T>(st.eta_trial * st.omega)};
+    epi.avg = it.avg_y;
+    epi.af = static_cast<T>(st.avg_frac);
+    epi.upd = st.avg_pending;
     row_block(a, bm, it.x[p ^ 1], lds, shT, epi);
     double v[2] = {epi.mov, epi.inter};
     block_sums<2>(v, sh);
@@ -255,8 +275,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* states, con
     const int k = blockIdx.x;
     DevState* st = states + k;
     if (threadIdx.x == 0) {
-        st->avg_pending = 0;
         stop = st->done;
+        // The primal and dual halves of this trial applied the previous
+        // acceptance to the averages; a finished instance's last one is
+        // applied by the flush after the launch sequence.
+        if (!stop) st->avg_pending = 0;
     }
     __syncthreads();
     if (stop) return;
@@ -340,6 +363,9 @@ __global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* states, con
 }
 
 // avg += f * (x - avg) over the freshly committed iterate (iterate_average.cpp:14-24).
+// Inside a launch sequence the next trial's primal and dual epilogues apply
+// it (fused, the same arithmetic); this kernel flushes what is pending at the
+// end of a sequence, before the host reads the averages.
 template <typename T>
 __global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict__ states, Iterates<T> it,
                                                       const Seg* __restrict__ segs, int ncs) {
@@ -354,6 +380,12 @@ __global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict
         const T a = avg[q];
         avg[q] = a + f * (cur[q] - a);
     }
+}
+
+// Clears the pending-average flags after a flush.
+__global__ void k_clear_avg(DevState* states, int count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) states[k].avg_pending = 0;
 }
 
 // Plain row sums out = M v (debug products, power iteration).
@@ -371,6 +403,84 @@ __global__ __launch_bounds__(kThreads) void k_spmv(Orient<V> M, const T* __restr
     __shared__ T shT[kWaves];
     StoreEpi<T> epi{out};
     row_block(M, M.meta[blockIdx.x], v, lds, shT, epi);
+}
+
+// ---------------------------------------------------------------------------
+// Row-sliced primal half.  For an LP whose y does not fit one XCD's L2, A^T y
+// runs over a reordered copy of A^T whose rows are (row slice, column) pairs
+// in slice-major order: the entries of column j whose row lies in slice s.
+// Its blocks hold up to kTileRowsMany rows and kTileNnz nonzeros.
+// All blocks in flight then gather from the same slice of y, which stays
+// L2-resident; each block writes the partial sums of its pairs.  The combine
+// pass sums each column's partials in slice order and runs the primal prox.
+// ---------------------------------------------------------------------------
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_primal_slices(Orient<V> sat, Iterates<T> it,
+                                                            const DevState* __restrict__ states,
+                                                            T* __restrict__ part) {
+    const int4 bm = sat.meta[blockIdx.x];
+    const DevState& st = states[sat.binst[blockIdx.x]];
+    if (st.done || st.phase != PhaseOf<V, T>::value) return;
+    __shared__ T lds[kTileNnz];
+    struct {
+        T* out;
+        __device__ void apply(int r, T s) { out[r] = s; }
+    } epi{part};
+    row_block_many(sat, bm, it.y[st.parity], lds, epi);
+}
+
+// One column per thread over segments of at most kThreads columns:
+// g_j = c_j + the column's slice partials (cidx[cptr[j] .. cptr[j+1]), summed
+// in slice order), then the primal prox and sum dx^2 / T as k_primal does
+// (pdhg_kernel.cpp:47-104).  The partial indices of a column are loaded
+// together, then the partials, so each thread waits on two rounds of loads.
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_primal_combine(const Seg* __restrict__ segs, ColData<T> cd,
+                                                             Iterates<T> it, const DevState* __restrict__ states,
+                                                             const int32_t* __restrict__ cptr,
+                                                             const int32_t* __restrict__ cidx,
+                                                             const T* __restrict__ part,
+                                                             double* __restrict__ part_p) {
+    const Seg sg = segs[blockIdx.x];
+    const DevState& st = states[sg.inst];
+    if (st.done || st.phase != PhaseOf<V, T>::value) return;
+    __shared__ double sh[kWaves];
+    const int p = st.parity;
+    PrimalEpi<T> epi{cd, it.x[p], it.x[p ^ 1], static_cast<T>(st.eta_trial / st.omega)};
+    epi.avg = it.avg_x;
+    epi.af = static_cast<T>(st.avg_frac);
+    epi.upd = st.avg_pending;
+    const int j = sg.begin + static_cast<int>(threadIdx.x);
+    if (j < sg.end) {
+        const typename PrimalEpi<T>::Pre pre = epi.load(j);
+        const int q0 = cptr[j], q1 = cptr[j + 1];
+        T sum = T(0);
+        for (int q = q0; q < q1; q += 8) {
+            int ci[8];
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (q + u < q1) ci[u] = cidx[q + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (q + u < q1) v[u] = part[ci[u]];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (q + u < q1) sum += v[u];
+        }
+        epi.apply(j, sum, pre);
+    }
+    double v[1] = {epi.acc};
+    block_sums<1>(v, sh);
+    if (threadIdx.x == 0) part_p[blockIdx.x] = v[0];
+}
+
+// dst[q] = src[perm[q]] (working values into the sliced order).
+template <typename T>
+__global__ void k_permute(int64_t n, const int32_t* __restrict__ perm, const T* __restrict__ src,
+                          T* __restrict__ dst) {
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += stride) dst[q] = src[perm[q]];
 }
 
 // A x for the current iterate of the instances with flag != 0 whose phase

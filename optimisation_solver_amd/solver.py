@@ -156,6 +156,16 @@ class Engine:
         ptr = [None if v is None else v.ctypes.data for v in keep]
         self._check(self.lib.pdlpx_update_vectors(self.ctx, index, *ptr), "pdlpx_update_vectors")
 
+    def set_slicing(self, rows_per_slice: int) -> None:
+        """Row-sliced primal half with `rows_per_slice` rows of A per slice (0 = off)."""
+        self._check(self.lib.pdlpx_set_slicing(self.ctx, rows_per_slice), "pdlpx_set_slicing")
+
+    def slicing(self) -> tuple[int, int]:
+        """(rows per slice or 0, number of (slice, column) rows)."""
+        a, b = C.c_int32(), C.c_int32()
+        self._check(self.lib.pdlpx_get_slicing(self.ctx, C.byref(a), C.byref(b)), "pdlpx_get_slicing")
+        return a.value, b.value
+
     def autotune(self, precision: int = _native.FP64, reps: int = 20) -> dict:
         """Times SpMVs of every candidate row-block plan of each orientation
         and keeps the fastest (pdlpx_autotune).  Run before a solve."""

@@ -1,0 +1,116 @@
+"""Row-sliced primal half (pdlpx_set_slicing): forced on small LPs with odd
+slice sizes, every precision mode and a batch.  The sliced engine must solve
+to the requested tolerance with the oracle's status and the GPU point,
+re-scored by the oracle's KKT evaluator, within tolerance; on the
+well-conditioned box family the objective also agrees to 1e-6 relative.  On
+the power-law family two valid 1e-6 points can differ by ~1e-5 in objective
+(the unsliced GPU solve does too), so there the bar is the KKT and 1e-4."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import problems
+from optimisation_solver_amd import Engine, box_lp, default_options, zipf_lp
+
+pytestmark = pytest.mark.gpu
+
+
+def _opts(tol=1e-6, **kw):
+    return default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, **kw)
+
+
+@pytest.mark.parametrize("rows_per_slice", [1, 37, 500, 100000])
+def test_sliced_solve_matches_oracle(rows_per_slice):
+    lp = box_lp(2000, 5000, 5, seed=61)
+    o = _opts()
+    with Engine() as e:
+        e.load(lp)
+        assert e.slicing()[0] == 0          # small LP: off by default
+        e.set_slicing(rows_per_slice)
+        rps, rows = e.slicing()
+        assert rps == rows_per_slice and rows >= np.count_nonzero(np.diff(lp.cp))
+        sliced = e.solve(o)
+        e.set_slicing(0)
+        assert e.slicing() == (0, 0)
+        plain = e.solve(o)
+    cpu = ol.solve(lp, o)
+    rel = abs(sliced.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective))
+    print(f"slices of {rows_per_slice} rows ({rows} slice rows): its {sliced.iterations} (unsliced "
+          f"{plain.iterations}, cpu {cpu.iterations}); relative objective difference {rel:.2e}", flush=True)
+    assert sliced.status == plain.status == cpu.status_name == "optimal"
+    assert rel <= 1e-6
+    m = ol.evaluate(lp, o, sliced.primal, sliced.row_dual)
+    assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+
+
+def test_sliced_long_columns_split_across_tiles():
+    # literal Zipf degrees: columns of up to 2000 entries; one slice of 10
+    # rows keeps them whole, so pieces of at most one tile are exercised with
+    # 1-row slices as well
+    lp = zipf_lp(3000, 7000, literal=True, seed=99)
+    assert np.diff(lp.cp).max() > 1000
+    o = _opts()
+    cpu = ol.solve(lp, o)
+    for rps in (100000, 1):
+        with Engine() as e:
+            e.load(lp)
+            e.set_slicing(rps)
+            r = e.solve(o)
+        rel = abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective))
+        print(f"long columns, {rps} rows per slice: {r.status} its {r.iterations}; objective diff {rel:.2e}",
+              flush=True)
+        assert r.status == cpu.status_name == "optimal"
+        m = ol.evaluate(lp, o, r.primal, r.row_dual)
+        assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+        assert rel <= 1e-4
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_sliced_reduced_precision(precision):
+    lp = box_lp(1500, 3750, 5, seed=62)
+    o = _opts(precision=precision)
+    with Engine() as e:
+        e.load(lp, precision)
+        e.set_slicing(211)
+        r = e.solve(o)
+    cpu = ol.solve(lp, _opts())
+    assert r.status == cpu.status_name == "optimal"
+    m = ol.evaluate(lp, o, r.primal, r.row_dual)
+    assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+    assert abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-4
+
+
+def test_sliced_batch_with_empty_and_invalid_members():
+    lps = [box_lp(400, 1000, 5, seed=63), problems.random_feasible(200, 300, 5), box_lp(900, 2250, 5, seed=64)]
+    bad = problems.lp(1, 1, [(0, 0, 1.0)], [1], [0], [1], [5], [1])
+    o = _opts(thread_count=1)
+    with Engine() as e:
+        e.load_batch(lps + [bad])
+        e.set_slicing(128)
+        res = e.solve_batch(o)
+    assert res[3].status == "invalid_problem"
+    for lp, r in zip(lps, res[:3]):
+        cpu = ol.solve(lp, o)
+        assert r.status == cpu.status_name
+        if r.status == "optimal":
+            assert abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-6
+            m = ol.evaluate(lp, o, r.primal, r.row_dual)
+            assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+
+
+def test_sliced_statuses_and_graph_replay():
+    inf = problems.infeasible_rows()
+    o = _opts(eager_launch=0)
+    with Engine() as e:
+        e.load(inf)
+        e.set_slicing(1)
+        r = e.solve(o)
+        assert r.status == "infeasible"
+    lp = box_lp(800, 2000, 5, seed=65)
+    with Engine() as e:
+        e.load(lp)
+        e.set_slicing(100)
+        g = e.solve(_opts(eager_launch=0))
+        q = e.solve(_opts(eager_launch=1))
+    assert (g.iterations, g.step_trials) == (q.iterations, q.step_trials)
+    assert np.array_equal(g.primal, q.primal) and np.array_equal(g.row_dual, q.row_dual)
