@@ -55,7 +55,12 @@ def trial_bytes(lp, s_v: int, s_x: int, s_d: int, s_i: int = 4, s_p: int = 4) ->
     return float(primal), float(dual)
 
 
-def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads: int) -> dict | None:
+def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads: int,
+                 budget_s: float = 10.0) -> dict | None:
+    """The reference PDLP (oracle/_ref, built from the reference sources) on
+    this host: set-up alone, then a bounded sample of iterations (about
+    `budget_s` of CPU work unless `sample_iters` fixes it), scaled to the
+    iteration count the GPU solve needed."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as ol
     from optimisation_solver_amd import default_options
@@ -72,7 +77,12 @@ def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads
         return time.perf_counter() - t, r
 
     t0, _ = run(0)
-    t1, r1 = run(sample_iters)
+    n = sample_iters
+    if n <= 0:   # size the sample to about budget_s of CPU work
+        tp, rp = run(200)
+        per = max(tp - t0, 1e-9) / max(rp.iterations, 1)
+        n = int(min(max(200, budget_s / per), max(gpu_iterations, 200)))
+    t1, r1 = run(n)
     per_iter = max(t1 - t0, 1e-9) / max(r1.iterations, 1)
     est = t0 + per_iter * gpu_iterations
     return {"value": round(est, 4), "unit": "s", "cores": used, "kind": kind,
@@ -80,7 +90,8 @@ def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads
                        f" on the same instance, {used} thread(s): set-up {t0:.3f}s (Ruiz+preconditioner+first KKT)"
                        f" + {r1.iterations} PDHG iterations in {t1 - t0:.3f}s = {per_iter * 1e3:.3f} ms/iteration,"
                        f" scaled to the {gpu_iterations} iterations the GPU solve needed"),
-            "ms_per_iteration": round(per_iter * 1e3, 4), "setup_s": round(t0, 4)}
+            "ms_per_iteration": round(per_iter * 1e3, 4), "setup_s": round(t0, 4),
+            "sample_iterations": int(r1.iterations), "sample_s": round(t1, 3)}
 
 
 def bench_batch(args) -> int:
@@ -283,7 +294,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--batch-count", type=int, default=256, help="instances of the batched workload (--config 3)")
-    ap.add_argument("--cpu-sample-iters", type=int, default=200)
+    ap.add_argument("--cpu-sample-iters", type=int, default=0,
+                    help="reference CPU iterations to time (0: about 10 s of CPU work)")
     ap.add_argument("--train-count", type=int, default=8000, help="training solves of the warm-start workload")
     ap.add_argument("--test-count", type=int, default=256, help="held-out instances of the warm-start workload")
     ap.add_argument("--train-tolerance", type=float, default=1e-4)
@@ -391,7 +403,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        log(f"[bench] cpu baseline sample ({args.cpu_sample_iters} iterations, {threads} threads)")
+        log(f"[bench] cpu baseline sample ({args.cpu_sample_iters or 'about 10 s of'} iterations, {threads} threads)")
         cpu = cpu_baseline(lp, tol, res.iterations, args.cpu_sample_iters, threads)
         log(f"[bench] cpu: {cpu}")
     eng.close()

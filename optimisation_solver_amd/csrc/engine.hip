@@ -2196,7 +2196,7 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
             h.target = std::numeric_limits<int64_t>::max();
         }
         HIPCHK(hipMemcpyAsync(X->st.p, probe.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
-        std::vector<hipEvent_t> ev(size_t(2) * reps + 1, nullptr);
+        std::vector<hipEvent_t> ev(size_t(3) * reps, nullptr);   // before primal, before dual, after dual
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         auto seq = [&](auto vtag, auto ttag) {
             using V = decltype(vtag);
@@ -2232,11 +2232,11 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
             dual();
             decide();
             for (int r = 0; r < reps; ++r) {
-                HIPCHK(hipEventRecord(ev[2 * r], s));
+                HIPCHK(hipEventRecord(ev[3 * r], s));
                 launchPrimalHalf<V, T>(*X, s, vc, wp->cols(), ip->view());
-                HIPCHK(hipEventRecord(ev[2 * r + 1], s));
+                HIPCHK(hipEventRecord(ev[3 * r + 1], s));
                 dual();
-                HIPCHK(hipEventRecord(ev[2 * r + 2], s));
+                HIPCHK(hipEventRecord(ev[3 * r + 2], s));
                 decide();
             }
         };
@@ -2248,8 +2248,8 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
         double tp = 0.0, td = 0.0;
         for (int r = 0; r < reps; ++r) {
             float a = 0.f, b = 0.f;
-            HIPCHK(hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]));
-            HIPCHK(hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]));
+            HIPCHK(hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
             tp += a;
             td += b;
         }
