@@ -1636,13 +1636,22 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     X.xOld.assign(X.n, 0);
     X.yNew.assign(X.m, 0);
     X.yOld.assign(X.m, 0);
+    // When the batch's x outgrows L2 (more than 262144 columns, 2 MB in
+    // FP64), columns of equal degree are further ordered by their first row
+    // in the sorted row order, so the many short columns that consecutive
+    // rows use sit next to each other: their gathers (x' in A x', y in A^T y)
+    // then share cache lines instead of touching one line each.  Smaller
+    // batches keep the plain degree order.
     std::vector<int32_t> order;
-    auto sortSpace = [&](int count, int off, const int64_t* start, std::vector<int32_t>& nw,
+    std::vector<int32_t> key;
+    auto sortSpace = [&](int count, int off, const int64_t* start, const int32_t* tie, std::vector<int32_t>& nw,
                          std::vector<int32_t>& old) {
         order.resize(count);
         for (int j = 0; j < count; ++j) order[j] = j;
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            return start[a + 1] - start[a] > start[b + 1] - start[b];
+            const int64_t da = start[a + 1] - start[a], db = start[b + 1] - start[b];
+            if (da != db) return da > db;
+            return tie && tie[a] < tie[b];
         });
         for (int q = 0; q < count; ++q) {
             old[off + q] = off + order[q];
@@ -1651,8 +1660,15 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     };
     for (int k = 0; k < B; ++k) {
         const pdlpx_problem& p = probs[X.dev[k]];
-        sortSpace(p.cols, X.colOff[k], p.csc_col_start, X.xNew, X.xOld);
-        sortSpace(p.rows, X.rowOff[k], p.csr_row_start, X.yNew, X.yOld);
+        sortSpace(p.rows, X.rowOff[k], p.csr_row_start, nullptr, X.yNew, X.yOld);
+        const bool byRow = X.n > 262144;
+        if (byRow) {
+            key.assign(size_t(p.cols), INT32_MAX);   // first row of each column in the sorted row order
+            for (int j = 0; j < p.cols; ++j)
+                for (int64_t e = p.csc_col_start[j]; e < p.csc_col_start[j + 1]; ++e)
+                    key[j] = std::min(key[j], X.yNew[X.rowOff[k] + p.csc_row[e]]);
+        }
+        sortSpace(p.cols, X.colOff[k], p.csc_col_start, byRow ? key.data() : nullptr, X.xNew, X.xOld);
     }
     std::vector<int32_t> rp(X.m + 1), cp(X.n + 1), ci(total), ri(total);
     std::vector<double> rv(total), cv(total), c(X.n), lo(X.n), hi(X.n), rlo(X.m), rhi(X.m);
