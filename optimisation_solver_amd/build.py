@@ -5,7 +5,9 @@
                                            kernels (hipcc)
   optimisation_solver_amd/liblpgen.so      instance generators (g++)
   tests/cpp/test_dropin                    C++ drop-in tests (g++, links the engine)
-  oracle/liboracle.so, oracle/_ref/...     CPU checker (make -C oracle)
+  oracle/liboracle.so, oracle/_ref/...     CPU checker (make -C oracle), and the
+                                           reference's unit tests built against
+                                           the drop-in (make -C oracle dropin)
 
 Run as ``python -m optimisation_solver_amd.build`` or through
 ``__graft_entry__.build()``.
@@ -92,8 +94,13 @@ def build_all() -> None:
     first = build_engine() + [build_lpgen()] + build_oracle()
     _wait(first)
     _wait([link_engine()])
+    later = []
     if (ROOT / "tests" / "cpp" / "test_dropin.cpp").exists():
-        _wait([build_dropin_test()])
+        later.append(build_dropin_test())
+    if Path("/root/reference/pdlp_engine/tests/test_pdlp.cpp").is_file():
+        # the reference's own unit tests against the drop-in headers and the engine
+        later.append(_run(["make", "-s", "-B", "-C", ROOT / "oracle", "dropin"]))
+    _wait(later)
 
 
 if __name__ == "__main__":
