@@ -100,6 +100,9 @@ def build_all() -> None:
     if Path("/root/reference/pdlp_engine/tests/test_pdlp.cpp").is_file():
         # the reference's own unit tests against the drop-in headers and the engine
         later.append(_run(["make", "-s", "-B", "-C", ROOT / "oracle", "dropin"]))
+    if Path("/root/reference/tests/pipeline/test_end_to_end.cpp").is_file():
+        # the reference's adapter and pipeline tests with PDLP served by the engine
+        later.append(_run(["make", "-s", "-B", "-j4", "-C", ROOT / "oracle", "pipeline"]))
     _wait(later)
 
 
