@@ -49,3 +49,33 @@ def test_reference_end_to_end_pipeline():
 def test_reference_postsolve_integration(case):
     r = _run("test_postsolve_integration_on_engine", case)
     assert r.returncode == 0 and f"[PASSED] {case}" in r.stdout
+
+
+def test_reference_validation_tool_answers_match_the_oracle(tmp_path):
+    """The reference's validation tool (tools/validate_against_reference.cpp,
+    unchanged) generates its six instance flavours (mixed, degenerate,
+    badly scaled, staircase, infeasible, unbounded), solves them at 1e-8
+    through pdlp::PdlpSolver (here: the GPU engine) and writes .lp.txt and
+    .answer.txt files.  Each written instance is read back and solved by the
+    oracle: same status, objective within 1e-6 relative, the answer's x
+    primal-feasible to the oracle's evaluator."""
+    import oracle_lib as ol
+    from optimisation_solver_amd import default_options
+    from optimisation_solver_amd.lpio import read_answer, read_lp_txt
+
+    r = _run("validate_on_engine", str(tmp_path), "6")
+    assert r.returncode == 0
+    o = default_options(primal_tolerance=1e-8, dual_tolerance=1e-8, gap_tolerance=1e-8, thread_count=1,
+                        iteration_limit=300000, time_limit_seconds=60.0)
+    for i in range(6):
+        lp = read_lp_txt(str(tmp_path / f"instance_{i}.lp.txt"))
+        ans = read_answer(str(tmp_path / f"instance_{i}.answer.txt"))
+        cpu = ol.solve(lp, o)
+        print(f"instance {i}: gpu {ans.status} {ans.primal_objective:.10g}; cpu {cpu.status_name} "
+              f"{cpu.primal_objective:.10g}", flush=True)
+        assert ans.status == cpu.status_name
+        if ans.status == "optimal":
+            assert abs(ans.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-6
+            import numpy as np
+            m = ol.evaluate(lp, o, ans.primal, np.zeros(lp.m))
+            assert m["primal_residual"] <= 1e-8
