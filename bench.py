@@ -398,6 +398,12 @@ def main() -> int:
     parity = {"oracle_kkt": ev["kkt"], "oracle_primal_residual": ev["primal_residual"],
               "oracle_dual_residual": ev["dual_residual"], "oracle_gap": ev["relative_gap"],
               "objective": res.primal_objective}
+    # the reference verifier's frozen criteria, recomputed from the original LP
+    from optimisation_solver_amd.verify import check_solution
+    vr = check_solution(lp, res.primal, res.row_dual)
+    verify = {"verdict": vr["verdict"], "max_row_violation_scaled": vr["primal"]["max_row_violation_scaled"],
+              "gap_normalised": vr["dual"].get("gap_normalised"),
+              "dual_residual_normalised": vr["dual"].get("dual_residual_normalised")}
 
     eng_tiling = eng.tiling()
     cpu = None
@@ -420,7 +426,7 @@ def main() -> int:
                    "setup_s": round(res.diagnostics["setup_seconds"], 4), "seed": lp.name,
                    "load_s": round(load_s, 4),
                    "tiling": tiling or eng_tiling},
-        "roofline": roof, "cpu_baseline": cpu, "parity": parity,
+        "roofline": roof, "cpu_baseline": cpu, "parity": parity, "verify": verify,
         "capped": args.iteration_limit < 10_000_000,
     }
     if cpu:
