@@ -19,6 +19,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -353,6 +354,28 @@ void mixedAndAutotuned() {
          "mixed precision, tuned plan objective");
 }
 
+// Re-entrancy: one device context per call, so solves from several host
+// threads at once must each equal the same solve run alone (bitwise: the
+// engine is deterministic).
+void concurrentSolves() {
+    std::vector<pdlp::CompiledLp> lps;
+    for (int k = 0; k < 4; ++k) lps.push_back(plantedLp(200 + 40 * k, 300 + 50 * k, 100u + unsigned(k)));
+    pdlp::PdlpOptions o;
+    o.primalTolerance = o.dualTolerance = o.gapTolerance = 1e-7;
+    std::vector<pdlp::PdlpResult> serial, together(lps.size());
+    for (const auto& lp : lps) serial.push_back(pdlp::PdlpSolver{}.solve(lp, o));
+    std::vector<std::thread> threads;
+    for (size_t k = 0; k < lps.size(); ++k)
+        threads.emplace_back([&, k] { together[k] = pdlp::PdlpSolver{}.solve(lps[k], o); });
+    for (auto& t : threads) t.join();
+    for (size_t k = 0; k < lps.size(); ++k) {
+        expect(serial[k].status == pdlp::PdlpStatus::Optimal, "serial solve optimal");
+        expect(together[k].status == serial[k].status && together[k].iterations == serial[k].iterations &&
+                   together[k].primal == serial[k].primal && together[k].rowDual == serial[k].rowDual,
+               "a concurrent solve equals the same solve run alone");
+    }
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -363,7 +386,7 @@ int main(int argc, char** argv) {
         {"degenerate", degenerate}, {"invalidRejected", invalidRejected}, {"timeLimit", timeLimit},
         {"linesearchAndPolicies", linesearchAndPolicies}, {"rays", rays}, {"feasibleNotFlagged", feasibleNotFlagged},
         {"warmStartNeverWorse", warmStartNeverWorse}, {"fp32Mode", fp32Mode},
-        {"mixedAndAutotuned", mixedAndAutotuned},
+        {"mixedAndAutotuned", mixedAndAutotuned}, {"concurrentSolves", concurrentSolves},
     };
     int failures = 0;
     for (const auto& [name, fn] : tests) {
