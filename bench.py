@@ -55,6 +55,18 @@ def trial_bytes(lp, s_v: int, s_x: int, s_d: int, s_i: int = 4, s_p: int = 4) ->
     return float(primal), float(dual)
 
 
+def gather_floor(eng, prec: int, reps: int, ms_p: float, ms_d: float) -> dict:
+    """Bare gather passes over the same nonzeros as the two halves (index and
+    value loads plus the gathered element, the trial kernels' blocks, nothing
+    else): the floor set by the random-gather rate, beside the fused halves."""
+    g_rows, g_cols = eng.time_gathers(prec, reps)
+    return {"dual_ms": round(g_rows, 5), "primal_ms": round(g_cols, 5),
+            "dual_frac": round(g_rows / ms_d, 3), "primal_frac": round(g_cols / ms_p, 3),
+            "note": "bare gather pass (index, value, gathered element) over each half's nonzeros with the "
+                    "trial kernels' row blocks; frac = floor / fused half (the sliced primal half also "
+                    "runs its combine pass)"}
+
+
 def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads: int,
                  budget_s: float = 10.0) -> dict | None:
     """The reference PDLP (oracle/_ref, built from the reference sources) on
@@ -144,6 +156,7 @@ def bench_batch(args) -> int:
     roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom[0],
             "bytes_per_launch": int(dom[2]), "ms_per_launch": round(dom[1], 5)}
+    roof["gather_floor"] = gather_floor(eng, prec, args.kernel_reps, ms_p, ms_d)
     eng.close()
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as ol
@@ -390,6 +403,7 @@ def main() -> int:
             "other_kernel": {"name": "pdhg_dual_kernel" if dom[0] == "pdhg_primal_kernel" else "pdhg_primal_kernel",
                              "ms_per_launch": round(ms_d if dom[0] == "pdhg_primal_kernel" else ms_p, 5),
                              "achieved": round(gbs_d if dom[0] == "pdhg_primal_kernel" else gbs_p, 1)}}
+    roof["gather_floor"] = gather_floor(eng, prec, args.kernel_reps, ms_p, ms_d)
 
     # parity: re-score the GPU point with the oracle's KKT evaluator
     sys.path.insert(0, str(ROOT / "tests"))

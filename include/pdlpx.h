@@ -236,6 +236,20 @@ int pdlpx_time_trial(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
                      double* ms_primal, double* ms_dual);
 
 /*
+ * Times a bare gather pass over each orientation: per nonzero only the index
+ * and value loads and the gather of the vector element, with the row blocks
+ * and thread mapping of the trial kernels but no row sums, prox, state or
+ * partials.  ms_rows: over A (gathers x, as the dual half does); ms_cols:
+ * over A^T, or its row-sliced copy when the load is sliced (gathers y, as the
+ * primal half does).  Average milliseconds over `reps` passes, hipEvents on
+ * the context's stream.  It is the floor the fused halves are measured
+ * against; the iterates are read, never written.  Needs a prior solve in
+ * `precision`.  A profiling aid with no reference counterpart.
+ */
+int pdlpx_time_gathers(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
+                       double* ms_rows, double* ms_cols);
+
+/*
  * Progress reporting: during pdlpx_solve / pdlpx_solve_loaded the engine calls
  * `fn(user, running_instances, max_iteration, best_kkt_of_instance_0)` from
  * the calling thread at a termination check, at most once every

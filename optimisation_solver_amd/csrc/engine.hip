@@ -2277,4 +2277,64 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
     });
 }
 
+int pdlpx_time_gathers(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_rows, double* ms_cols) {
+    if (!ms_rows || !ms_cols || reps <= 0) return PDLPX_E_INVALID;
+    if (precision != PDLPX_FP64 && precision != PDLPX_FP32 && precision != PDLPX_MIXED) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        if (precision != PDLPX_FP64 && !X->have32) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
+        hipStream_t s = X->stream;
+        const OrientBuf& C = X->sliced ? X->SAT : X->AT;
+        DBuf<double> out;
+        out.alloc(size_t(std::max(X->A.nblk, C.nblk)) * kWaves);
+        hipEvent_t e[3];
+        for (auto& h : e) HIPCHK(hipEventCreate(&h));
+        double tr = 0.0, tc = 0.0;
+        auto run = [&](auto vtag, auto ttag) {
+            using V = decltype(vtag);
+            using T = decltype(ttag);
+            const V* vr;
+            const V* vcol;
+            if constexpr (sizeof(V) == 4) {
+                vr = X->w32.vr.p;
+                vcol = X->sliced ? X->satV32.p : X->w32.vc.p;
+            } else {
+                vr = X->w64.vr.p;
+                vcol = X->sliced ? X->satV64.p : X->w64.vc.p;
+            }
+            const IterBuf<T>* ip;
+            if constexpr (sizeof(T) == 4) ip = &X->it32;
+            else ip = &X->it64;
+            if (!vr || !vcol || !ip->x[0].p || !ip->y[0].p)
+                throw Failure{PDLPX_E_STATE, "no solve in this precision has run"};
+            T* o = reinterpret_cast<T*>(out.p);
+            auto rows = [&] { k_gather_probe<V, T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<V>(vr), ip->x[0].p, o); };
+            auto cols = [&] { k_gather_probe<V, T><<<C.nblk, kThreads, 0, s>>>(C.view<V>(vcol), ip->y[0].p, o); };
+            rows();
+            cols();
+            for (int r = 0; r < reps; ++r) {
+                HIPCHK(hipEventRecord(e[0], s));
+                rows();
+                HIPCHK(hipEventRecord(e[1], s));
+                cols();
+                HIPCHK(hipEventRecord(e[2], s));
+                HIPCHK(hipEventSynchronize(e[2]));
+                float a = 0.f, b = 0.f;
+                HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+                HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+                tr += a;
+                tc += b;
+            }
+        };
+        if (precision == PDLPX_FP32) run(float{}, float{});
+        else if (precision == PDLPX_MIXED) run(float{}, double{});
+        else run(double{}, double{});
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        for (auto& h : e) (void)hipEventDestroy(h);
+        *ms_rows = tr / reps;
+        *ms_cols = tc / reps;
+    });
+}
+
 }  // extern "C"

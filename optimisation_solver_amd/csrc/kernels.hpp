@@ -263,103 +263,120 @@ __device__ inline double dclamp(double v, double lo, double hi) {   // std::clam
 
 // Linesearch decision and commit, one workgroup per instance
 // (step_controller.cpp:30-68, pdlp_solver.cpp:292-348,
-// iterate_average.cpp:205-247).  Eight independent accumulators per thread
-// keep many partial loads in flight.
+// iterate_average.cpp:205-247).  It sits on the critical path between two
+// trials, so its dependent memory round trips are kept few: the state and the
+// partial ranges are read together (a finished instance only skips the
+// work), then the first kDecideP primal and kDecideD dual partial slots of
+// every thread are all loaded before any is summed; ranges beyond
+// kReduceThreads times those counts continue in a plain loop.  The
+// step-schedule powers are computed while the loads are in flight.  Each
+// thread sums its partials in increasing index order and the workgroup tree
+// combines the threads, so the sums are the same run to run.
+constexpr int kDecideP = 8;
+constexpr int kDecideD = 12;
+
 __global__ __launch_bounds__(kReduceThreads) void k_decide(DevState* states, const int32_t* __restrict__ pbeg,
                                                             const int32_t* __restrict__ dbeg,
                                                             const double* __restrict__ pp,
                                                             const double* __restrict__ pd,
                                                             const double* __restrict__ pi) {
     __shared__ double sh[3 * (kReduceThreads / 64)];
-    __shared__ int stop;
     const int k = blockIdx.x;
     DevState* st = states + k;
-    if (threadIdx.x == 0) {
-        stop = st->done;
-        // The primal and dual halves of this trial applied the previous
-        // acceptance to the averages; a finished instance's last one is
-        // applied by the flush after the launch sequence.
-        if (!stop) st->avg_pending = 0;
-    }
-    __syncthreads();
-    if (stop) return;
-    constexpr int U = 8;
-    double ap[U] = {}, ad[U] = {}, ai[U] = {};
+    const DevState s0 = *st;
     const int p0 = pbeg[k], p1 = pbeg[k + 1], d0 = dbeg[k], d1 = dbeg[k + 1];
-    for (int i = p0 + threadIdx.x; i < p1; i += U * kReduceThreads) {
+    const int live = s0.done == 0;
+    const int np = live * (p1 - p0);
+    const int nd = live * (d1 - d0);
+    const int t = static_cast<int>(threadIdx.x);
+    double rp[kDecideP], rd[kDecideD], ri[kDecideD];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = i + u * kReduceThreads;
-            if (q < p1) ap[u] += pp[q];
-        }
+    for (int u = 0; u < kDecideP; ++u) {
+        const int q = t + u * kReduceThreads;
+        rp[u] = q < np ? pp[p0 + q] : 0.0;
     }
-    for (int i = d0 + threadIdx.x; i < d1; i += U * kReduceThreads) {
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = i + u * kReduceThreads;
-            if (q < d1) {
-                ad[u] += pd[q];
-                ai[u] += pi[q];
-            }
-        }
+    for (int u = 0; u < kDecideD; ++u) {
+        const int q = t + u * kReduceThreads;
+        rd[u] = q < nd ? pd[d0 + q] : 0.0;
+        ri[u] = q < nd ? pi[d0 + q] : 0.0;
     }
-    double v[3] = {0.0, 0.0, 0.0};
+    // the k+2 schedule of the step controller (step_controller.cpp:43-47)
+    const double kk = static_cast<double>(s0.iteration) + 2.0;
+    double shrink = 0.0, grow = 0.0;
+    if (t == 0 && s0.adaptive && !s0.done) {
+        shrink = 1.0 - pow(kk, -0.3);
+        grow = 1.0 + pow(kk, -0.6);
+    }
+    double ap = rp[0], ad = rd[0], ai = ri[0];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        v[0] += ap[u];
-        v[1] += ad[u];
-        v[2] += ai[u];
+    for (int u = 1; u < kDecideP; ++u) ap += rp[u];
+#pragma unroll
+    for (int u = 1; u < kDecideD; ++u) {
+        ad += rd[u];
+        ai += ri[u];
     }
+    for (int q = t + kDecideP * kReduceThreads; q < np; q += kReduceThreads) ap += pp[p0 + q];
+    for (int q = t + kDecideD * kReduceThreads; q < nd; q += kReduceThreads) {
+        ad += pd[d0 + q];
+        ai += pi[d0 + q];
+    }
+    if (s0.done) return;
+    double v[3] = {ap, ad, ai};
     group_sums<kReduceThreads, 3>(v, sh);
     if (threadIdx.x != 0) return;
+    DevState s = s0;
+    // The primal and dual halves of this trial applied the previous
+    // acceptance to the averages; a finished instance's last one is applied
+    // by the flush after the launch sequence.
+    s.avg_pending = 0;
     const double pm = v[0], dm = v[1], inter = v[2];
-    st->pm = pm;
-    st->dm = dm;
-    st->inter = inter;
-    st->trials += 1;
+    s.pm = pm;
+    s.dm = dm;
+    s.inter = inter;
+    s.trials += 1;
     if (!(isfinite(pm) && isfinite(dm) && isfinite(inter))) {
-        st->fail = 1;
-        st->done = 1;
+        s.fail = 1;
+        s.done = 1;
+        *st = s;
         return;
     }
     bool ok = true;
-    if (st->adaptive) {
-        const double omega = st->omega;
+    if (s.adaptive) {
+        const double omega = s.omega;
         const double mov = omega * pm + dm / omega;
         const double cpl = fabs(inter);
         double limit = INFINITY;
         if (cpl > 0.0 && mov > 0.0) limit = mov / (2.0 * cpl);
-        const double kk = static_cast<double>(st->iteration) + 2.0;
-        const double shrink = 1.0 - pow(kk, -0.3);
-        const double grow = 1.0 + pow(kk, -0.6);
-        ok = st->eta <= limit;
-        double next = smin(shrink * limit, grow * st->eta);
-        next = dclamp(next, st->min_step, 1e12);
-        if (!isfinite(next) || next <= 0.0) next = st->eta;
-        st->eta = next;
-        st->eta_trial = next;
+        ok = s.eta <= limit;
+        double next = smin(shrink * limit, grow * s.eta);
+        next = dclamp(next, s.min_step, 1e12);
+        if (!isfinite(next) || next <= 0.0) next = s.eta;
+        s.eta = next;
+        s.eta_trial = next;
     }
     if (ok) {
-        st->parity ^= 1;
-        st->iteration += 1;
-        st->attempts = 0;
-        if (st->averaging) {
-            const double w = st->eta_trial;
+        s.parity ^= 1;
+        s.iteration += 1;
+        s.attempts = 0;
+        if (s.averaging) {
+            const double w = s.eta_trial;
             if (w > 0.0) {
-                const double total = st->avg_total + w;
-                st->avg_frac = w / total;
-                st->avg_total = total;
-                st->avg_pending = 1;
+                const double total_w = s.avg_total + w;
+                s.avg_frac = w / total_w;
+                s.avg_total = total_w;
+                s.avg_pending = 1;
             }
         }
-        if (st->iteration >= st->target) st->done = 1;
+        if (s.iteration >= s.target) s.done = 1;
     } else {
-        st->attempts += 1;
-        if (st->attempts >= st->max_trials) {
-            st->fail = 2;
-            st->done = 1;
+        s.attempts += 1;
+        if (s.attempts >= s.max_trials) {
+            s.fail = 2;
+            s.done = 1;
         }
     }
+    *st = s;
 }
 
 // avg += f * (x - avg) over the freshly committed iterate (iterate_average.cpp:14-24).
@@ -386,6 +403,39 @@ __global__ __launch_bounds__(kThreads) void k_average(const DevState* __restrict
 __global__ void k_clear_avg(DevState* states, int count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < count) states[k].avg_pending = 0;
+}
+
+// Bare gather pass over the nonzeros of each row block: the index and value
+// loads and the gather of v, with the blocks and thread mapping of the trial
+// kernels (stage_chunk) but no LDS row sums, prox, state or partials; each
+// wave writes one sum so the loads stay live.  The floor a fused trial kernel
+// over the same orientation is measured against.
+template <typename V, typename T>
+__global__ __launch_bounds__(kThreads) void k_gather_probe(Orient<V> M, const T* __restrict__ v,
+                                                           T* __restrict__ out) {
+    const int4 bm = M.meta[blockIdx.x];
+    const int t = threadIdx.x;
+    T acc = T(0);
+    for (int k0 = bm.z; k0 < bm.w; k0 += kTileNnz) {
+        const int k1 = min(bm.w, k0 + kTileNnz);
+        int col[kPerThread];
+        V a[kPerThread];
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int k = k0 + t + q * kThreads;
+            if (k < k1) {
+                col[q] = stream_load(M.idx + k);
+                a[q] = stream_load(M.val + k);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < kPerThread; ++q) {
+            const int k = k0 + t + q * kThreads;
+            if (k < k1) acc += static_cast<T>(a[q]) * v[col[q]];
+        }
+    }
+    acc = wave_sum(acc);
+    if ((t & 63) == 0) out[size_t(blockIdx.x) * kWaves + (t >> 6)] = acc;
 }
 
 // Plain row sums out = M v (debug products, power iteration).

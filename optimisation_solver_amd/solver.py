@@ -149,6 +149,15 @@ class Engine:
                     "pdlpx_time_trial")
         return a.value, b.value
 
+    def time_gathers(self, precision: int = _native.FP64, reps: int = 20) -> tuple[float, float]:
+        """Average ms of a bare gather pass over A (x gathers, the dual half's
+        access pattern) and over A^T or its sliced copy (y gathers, the primal
+        half's): the floor the fused trial kernels are measured against."""
+        a, b = C.c_double(), C.c_double()
+        self._check(self.lib.pdlpx_time_gathers(self.ctx, precision, reps, C.byref(a), C.byref(b)),
+                    "pdlpx_time_gathers")
+        return a.value, b.value
+
     def update_vectors(self, index: int, c=None, rlo=None, rhi=None) -> None:
         """New objective and/or row bounds for loaded instance `index`; the
         matrix stays on the device (pdlpx_update_vectors)."""

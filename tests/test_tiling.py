@@ -92,3 +92,28 @@ def test_invalid_tiling_rejected():
         for bad in [(16, 64, 256, 64), (4096, 64, 256, 64), (256, 0, 256, 64), (256, 64, 256, 257)]:
             with pytest.raises(EngineError, match="tile limits"):
                 e.set_tiling(*bad)
+
+
+@pytest.mark.parametrize("sliced", [False, True])
+def test_time_trial_and_gather_floor(sliced):
+    """The profiling hooks: both return positive times on a solved load (the
+    sliced copy is probed when the load is sliced); the bare gather pass
+    reads the iterate and leaves it unchanged, so a solve after the probe
+    equals a solve before it; FP32 probes need an FP32 solve first."""
+    lp = zipf_lp(3000, 7000, seed=5)
+    o = default_options(primal_tolerance=1e-6, dual_tolerance=1e-6, gap_tolerance=1e-6)
+    print(f"sliced {sliced}", flush=True)
+    with Engine() as e:
+        e.load(lp)
+        if sliced:
+            e.set_slicing(512)
+            assert e.slicing()[0] == 512
+        r0 = e.solve(o)
+        with pytest.raises(EngineError):
+            e.time_gathers(1, 3)
+        g_rows, g_cols = e.time_gathers(0, 5)
+        t_p, t_d = e.time_trial(0, 5)
+        assert 0 < g_rows < 1e3 and 0 < g_cols < 1e3 and 0 < t_p < 1e3 and 0 < t_d < 1e3
+        r1 = e.solve(o)
+    assert r1.iterations == r0.iterations
+    assert np.array_equal(r1.primal, r0.primal) and np.array_equal(r1.row_dual, r0.row_dual)
