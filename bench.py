@@ -328,6 +328,9 @@ def main() -> int:
                     help="launch trial kernels eagerly instead of through the captured hipGraph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run device copies of MB megabytes (calibrates rocprofv3 FETCH/WRITE counters)")
+    ap.add_argument("--gather-rates", action="store_true",
+                    help="only measure the device's random-gather rate against table size (4- and 8-byte "
+                         "elements, 21.3M gathers per pass) and print one JSON line")
     ap.add_argument("--iteration-limit", type=int, default=10_000_000,
                     help="cap on PDHG iterations (profiling runs; a capped run is not a time-to-tolerance)")
     args = ap.parse_args()
@@ -340,6 +343,20 @@ def main() -> int:
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 10}), flush=True)
+        return 0
+    if args.gather_rates:
+        from optimisation_solver_amd import Engine
+        eng = Engine(0)
+        sizes_kb = [16, 64, 256, 512, 1024, 2048, 4096, 8192, 16384, 65536, 262144]
+        rates = {}
+        for w in (4, 8):
+            for kb in sizes_kb:
+                g = eng.gather_rate(kb * 1024, w, reps=args.kernel_reps)
+                rates[f"{w}B_{kb}KB"] = round(g / 1e9, 1)
+                log(f"[bench] gather rate, {w}-byte elements, {kb} KB table: {g / 1e9:.1f} G/s")
+        eng.close()
+        print(json.dumps({"gather_rates_G_per_s": rates, "gathers_per_pass": 2048 * 10417,
+                          "reps": args.kernel_reps}), flush=True)
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "

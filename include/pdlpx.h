@@ -250,6 +250,19 @@ int pdlpx_time_gathers(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
                        double* ms_rows, double* ms_cols);
 
 /*
+ * Random-gather rate of the device: `gathers` (rounded down to a multiple of
+ * 2048) gathers of `elem_bytes` (4 or 8) elements from a table of
+ * `table_bytes` (at most 1 GiB) at uniformly hashed positions, each with its
+ * streamed index and value, in the trial kernels' thread layout; writes
+ * gathers per second over `reps` back-to-back passes.  Uses the context's
+ * device and stream only (no problem needs to be loaded).  The ceiling
+ * that pdlpx_time_gathers is read against.  A profiling aid with no
+ * reference counterpart.
+ */
+int pdlpx_gather_rate(pdlpx_ctx* ctx, int64_t table_bytes, int32_t elem_bytes, int64_t gathers, int32_t reps,
+                      double* gathers_per_second);
+
+/*
  * Progress reporting: during pdlpx_solve / pdlpx_solve_loaded the engine calls
  * `fn(user, running_instances, max_iteration, best_kkt_of_instance_0)` from
  * the calling thread at a termination check, at most once every

@@ -122,6 +122,7 @@ def engine() -> C.CDLL:
         lib.pdlpx_set_progress.argtypes = [vp, PROGRESS_FN, vp, f64]
         lib.pdlpx_time_trial.argtypes = [vp, i32, i32, pf64, pf64]
         lib.pdlpx_time_gathers.argtypes = [vp, i32, i32, pf64, pf64]
+        lib.pdlpx_gather_rate.argtypes = [vp, C.c_int64, i32, C.c_int64, i32, pf64]
         lib.pdlpx_solve_batch.argtypes = [vp, i32, C.POINTER(Problem), C.POINTER(Options),
                                           C.POINTER(Summary), vp, vp]
         lib.pdlpx_mlp_create.argtypes = [C.POINTER(vp), i32, i32, vp, i32, C.c_uint64]

@@ -141,22 +141,64 @@ struct OrientBuf {
     int32_t nblk = 0, nrows = 0;
     std::vector<int32_t> hptr;   // host copy of ptr, for re-planning
     int tileNnz = kTileNnz, tileRows = kTileRows;
+    // Host copies of the plan, and the compacted plan the trial kernels use
+    // while only some instances of a batch are still running (cnblk >= 0):
+    // the blocks of those instances, in plan order, with per-instance starts
+    // (empty ranges for the others).
+    std::vector<int4> hmeta;
+    std::vector<int32_t> hbinst, hibeg;
+    DBuf<int4> cmeta;
+    DBuf<int32_t> cbinst, cibeg;
+    int32_t cnblk = -1;
     template <typename T>
     Orient<T> view(const T* values) const {
         return Orient<T>{ptr.p, idx.p, values, meta.p, binst.p, nblk, nrows};
     }
+    // The view, block count and per-instance block starts of the trial kernels.
+    template <typename T>
+    Orient<T> tview(const T* values) const {
+        return cnblk < 0 ? view(values) : Orient<T>{ptr.p, idx.p, values, cmeta.p, cbinst.p, cnblk, nrows};
+    }
+    int32_t tnblk() const { return cnblk < 0 ? nblk : cnblk; }
+    const int32_t* tbeg() const { return cnblk < 0 ? ibeg.p : cibeg.p; }
     // Row-block plan for the given limits (blocks, owners, per-instance starts).
     void plan(const std::vector<int32_t>& instStart, int nnzLimit, int rowLimit, hipStream_t s) {
-        std::vector<int32_t> blk, bi, ib;
-        partitionRows(hptr, instStart, nnzLimit, rowLimit, blk, bi, ib);
-        nblk = static_cast<int32_t>(bi.size());
-        std::vector<int4> mt(nblk);
-        for (int b = 0; b < nblk; ++b) mt[b] = make_int4(blk[b], blk[b + 1], hptr[blk[b]], hptr[blk[b + 1]]);
-        meta.upload(mt.data(), mt.size(), s);
-        binst.upload(bi.data(), bi.size(), s);
-        ibeg.upload(ib.data(), ib.size(), s);
+        std::vector<int32_t> blk;
+        partitionRows(hptr, instStart, nnzLimit, rowLimit, blk, hbinst, hibeg);
+        nblk = static_cast<int32_t>(hbinst.size());
+        hmeta.resize(nblk);
+        for (int b = 0; b < nblk; ++b) hmeta[b] = make_int4(blk[b], blk[b + 1], hptr[blk[b]], hptr[blk[b + 1]]);
+        meta.upload(hmeta.data(), hmeta.size(), s);
+        binst.upload(hbinst.data(), hbinst.size(), s);
+        ibeg.upload(hibeg.data(), hibeg.size(), s);
         tileNnz = nnzLimit;
         tileRows = rowLimit;
+        cnblk = -1;
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    // Blocks of the instances with live[k] != 0.
+    int64_t liveBlocks(const std::vector<char>& live) const {
+        int64_t c = 0;
+        for (size_t k = 0; k < live.size(); ++k)
+            if (live[k]) c += hibeg[k + 1] - hibeg[k];
+        return c;
+    }
+    // Compacted plan over the instances with live[k] != 0 (at least one).
+    void compact(const std::vector<char>& live, hipStream_t s) {
+        std::vector<int4> m;
+        std::vector<int32_t> bi, ib(live.size() + 1, 0);
+        for (size_t k = 0; k < live.size(); ++k) {
+            if (live[k])
+                for (int32_t b = hibeg[k]; b < hibeg[k + 1]; ++b) {
+                    m.push_back(hmeta[b]);
+                    bi.push_back(hbinst[b]);
+                }
+            ib[k + 1] = static_cast<int32_t>(m.size());
+        }
+        cmeta.upload(m.data(), m.size(), s);
+        cbinst.upload(bi.data(), bi.size(), s);
+        cibeg.upload(ib.data(), ib.size(), s);
+        cnblk = static_cast<int32_t>(m.size());
         HIPCHK(hipStreamSynchronize(s));   // the local staging vectors die here
     }
 };
@@ -278,6 +320,17 @@ struct pdlpx_ctx {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
         graphs.clear();
     }
+
+    // Instances the compacted trial plan covers (empty: the full plan).
+    std::vector<char> compactSet;
+    // Back to the full plan (dropping graphs captured with the compacted one).
+    void resetCompaction() {
+        if (compactSet.empty()) return;
+        compactSet.clear();
+        A.cnblk = -1;
+        AT.cnblk = -1;
+        dropGraphs();
+    }
 };
 
 namespace {
@@ -342,13 +395,13 @@ void launchPrimalHalf(pdlpx_ctx& X, hipStream_t s, const V* colVals, ColData<T> 
         k_primal_combine<V, T><<<X.nccs, kThreads, 0, s>>>(X.csegs.p, cols, it, X.st.p, X.cptr.p, X.cidx.p,
                                                            satParts<T>(X), X.part_p.p);
     } else {
-        k_primal<V, T><<<X.AT.nblk, kThreads, 0, s>>>(X.AT.view<V>(colVals), cols, it, X.st.p, X.part_p.p);
+        k_primal<V, T><<<X.AT.tnblk(), kThreads, 0, s>>>(X.AT.tview<V>(colVals), cols, it, X.st.p, X.part_p.p);
     }
 }
 
 // Per-instance first partial of the primal half (row blocks of A^T, or
 // column segments when sliced).
-const int32_t* primalBegins(const pdlpx_ctx& X) { return X.sliced ? X.cSegBeg.p : X.AT.ibeg.p; }
+const int32_t* primalBegins(const pdlpx_ctx& X) { return X.sliced ? X.cSegBeg.p : X.AT.tbeg(); }
 
 // ===========================================================================
 // One batched solve.
@@ -478,7 +531,7 @@ private:
     }
     template <typename V, typename T>
     void launchDual() {
-        k_dual<V, T><<<X.A.nblk, kThreads, 0, s>>>(X.A.view<V>(rowValues<V>()), work<T>().rows(), iters<T>().view(),
+        k_dual<V, T><<<X.A.tnblk(), kThreads, 0, s>>>(X.A.tview<V>(rowValues<V>()), work<T>().rows(), iters<T>().view(),
                                                    X.st.p, X.part_d.p, X.part_i.p);
     }
     template <typename T>
@@ -492,7 +545,7 @@ private:
         if (phases & 1) launchDual<double, double>();
         if (phases & 2) launchDual<float, float>();
         if (phases & 4) launchDual<float, double>();
-        k_decide<<<B, kReduceThreads, 0, s>>>(X.st.p, primalBegins(X), X.A.ibeg.p, X.part_p.p, X.part_d.p,
+        k_decide<<<B, kReduceThreads, 0, s>>>(X.st.p, primalBegins(X), X.A.tbeg(), X.part_p.p, X.part_d.p,
                                               X.part_i.p);
     }
     // Applies the averaging update still pending after the last trial of a
@@ -522,6 +575,36 @@ private:
     // fails.  Pushes the state mirror first and pulls it at the end.  Every
     // slot either commits an iteration or counts a rejection toward the
     // linesearch budget, so the loop ends.
+    // Batch compaction: once the running instances hold at most half of the
+    // blocks the trial kernels launch, the trial plan is cut down to their
+    // blocks (graphs are captured again for it); an instance running again
+    // outside the compacted set widens it.  Finished instances' blocks would
+    // only exit at their first instruction, so results are unchanged.
+    void maybeCompact() {
+        if (B < 2 || X.sliced) return;
+        std::vector<char> live(B, 0);
+        int nlive = 0;
+        bool outside = false;
+        for (int k = 0; k < B; ++k)
+            if (!hs[k].done) {
+                live[k] = 1;
+                ++nlive;
+                if (!X.compactSet.empty() && !X.compactSet[k]) outside = true;
+            }
+        if (nlive == 0) return;
+        const int64_t now = int64_t(X.A.tnblk()) + X.AT.tnblk();
+        const int64_t need = X.A.liveBlocks(live) + X.AT.liveBlocks(live);
+        if (!outside && 2 * need > now) return;
+        if (2 * need > int64_t(X.A.nblk) + X.AT.nblk) {   // most of the batch: the full plan
+            X.resetCompaction();
+            return;
+        }
+        X.A.compact(live, s);
+        X.AT.compact(live, s);
+        X.compactSet = live;
+        X.dropGraphs();
+    }
+
     void runTrials() {
         pushState();
         int phases = 0;
@@ -532,6 +615,7 @@ private:
                 left = std::max<int64_t>(left, hs[k].target - hs[k].iteration);
             }
         if (!phases) return;
+        maybeCompact();
         const int big = o.termination_check_frequency + 4;
         while (true) {
             const int slots = static_cast<int>(std::min<int64_t>(big, left + 4 > 16 ? big : 16));
@@ -1403,6 +1487,7 @@ void Run::collect(pdlpx_summary* const* sums, double* const* xOut, double* const
 
 void Run::go(const double* const* warmX, const double* const* warmY, pdlpx_summary* const* sums,
              double* const* xOut, double* const* yOut, double* const* rayOut) {
+    X.resetCompaction();
     HIPCHK(hipEventRecord(X.ev0, s));
     setup();
     initStart(warmX, warmY);
@@ -1419,6 +1504,7 @@ void Run::go(const double* const* warmX, const double* const* warmY, pdlpx_summa
     loopSeconds = ms * 1e-3;
     polishStage();
     collect(sums, xOut, yOut, rayOut);
+    X.resetCompaction();
 }
 
 // ---------------------------------------------------------------------------
@@ -1584,6 +1670,7 @@ void allocWork(pdlpx_ctx& X, bool fp32) {
 }
 
 void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t precision) {
+    X.resetCompaction();
     X.dropGraphs();
     X.loaded = true;
     X.info.assign(count, InstInfo{});
@@ -1954,6 +2041,7 @@ int pdlpx_set_slicing(pdlpx_ctx* X, int32_t rows_per_slice) {
         std::vector<int32_t> ri(static_cast<size_t>(X->nnz));
         if (X->nnz) HIPCHK(hipMemcpyAsync(ri.data(), X->AT.idx.p, sizeof(int32_t) * ri.size(), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        X->resetCompaction();
         X->dropGraphs();
         X->sliceRows = rows_per_slice;
         buildSlices(*X, X->AT.hptr, ri, s);
@@ -2090,6 +2178,7 @@ int pdlpx_autotune(pdlpx_ctx* X, int32_t precision, int32_t reps, pdlpx_tiling* 
         const int R = reps > 0 ? reps : 20;
         hipStream_t s = X->stream;
         if (precision != PDLPX_FP64 && !X->have32) allocWork(*X, true);
+        X->resetCompaction();
         X->dropGraphs();
         pdlpx_tiling t{};
         const int n = X->n, m = X->m;
@@ -2139,6 +2228,7 @@ int pdlpx_set_tiling(pdlpx_ctx* X, const pdlpx_tiling* t) {
     }
     return guarded(X, [&] {
         if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
+        X->resetCompaction();
         X->dropGraphs();
         X->A.plan(X->rowOff, t->row_tile_nnz, t->row_tile_rows, X->stream);
         X->AT.plan(X->colOff, t->col_tile_nnz, t->col_tile_rows, X->stream);
@@ -2199,6 +2289,7 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
     return guarded(X, [&] {
         if (!X->loaded || X->B == 0) throw Failure{PDLPX_E_STATE, "no valid problem loaded"};
         if (precision != PDLPX_FP64 && !X->have32) throw Failure{PDLPX_E_STATE, "no FP32 solve has run"};
+        X->resetCompaction();
         hipStream_t s = X->stream;
         const int B = X->B;
         std::vector<DevState> saved(B), probe;
@@ -2334,6 +2425,44 @@ int pdlpx_time_gathers(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms
         for (auto& h : e) (void)hipEventDestroy(h);
         *ms_rows = tr / reps;
         *ms_cols = tc / reps;
+    });
+}
+
+int pdlpx_gather_rate(pdlpx_ctx* X, int64_t table_bytes, int32_t elem_bytes, int64_t gathers, int32_t reps,
+                      double* gathers_per_second) {
+    if (!gathers_per_second || reps <= 0 || (elem_bytes != 4 && elem_bytes != 8)) return PDLPX_E_INVALID;
+    if (table_bytes < elem_bytes || table_bytes > (int64_t(1) << 30)) return PDLPX_E_INVALID;
+    if (gathers < kTileNnz || gathers > (int64_t(1) << 28)) return PDLPX_E_INVALID;
+    return guarded(X, [&] {
+        hipStream_t s = X->stream;
+        const int64_t n = (gathers / kTileNnz) * kTileNnz;
+        const int blocks = static_cast<int>(n / kTileNnz);
+        const uint32_t elems = static_cast<uint32_t>(table_bytes / elem_bytes);
+        DBuf<int32_t> idx;
+        DBuf<float> val;
+        DBuf<double> tab, out;
+        idx.alloc(size_t(n));
+        val.alloc(size_t(n));
+        tab.alloc(size_t((table_bytes + 7) / 8));
+        out.alloc(size_t(blocks) * kWaves);
+        HIPCHK(hipMemsetAsync(tab.p, 0, tab.n * sizeof(double), s));
+        k_fill_gather_input<<<gridFor(n), kThreads, 0, s>>>(idx.p, val.p, n, elems, 0x5EEDull);
+        auto launch = [&] {
+            if (elem_bytes == 4)
+                k_gather_table<float><<<blocks, kThreads, 0, s>>>(idx.p, val.p, reinterpret_cast<const float*>(tab.p),
+                                                                  reinterpret_cast<float*>(out.p));
+            else
+                k_gather_table<double><<<blocks, kThreads, 0, s>>>(idx.p, val.p, tab.p, out.p);
+        };
+        launch();
+        HIPCHK(hipEventRecord(X->ev0, s));
+        for (int r = 0; r < reps; ++r) launch();
+        HIPCHK(hipEventRecord(X->ev1, s));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventSynchronize(X->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, X->ev0, X->ev1));
+        *gathers_per_second = double(n) * reps / (double(ms) * 1e-3);
     });
 }
 

@@ -438,6 +438,45 @@ __global__ __launch_bounds__(kThreads) void k_gather_probe(Orient<V> M, const T*
     if ((t & 63) == 0) out[size_t(blockIdx.x) * kWaves + (t >> 6)] = acc;
 }
 
+// Random-gather rate probe: `n` gathers (n a multiple of kTileNnz) from a
+// table of `elems` entries at uniformly hashed positions, with the trial
+// kernels' thread layout (kPerThread per thread, index and value streamed).
+__device__ inline uint32_t hash_index(uint64_t i, uint64_t seed, uint32_t elems) {
+    uint64_t z = i + seed * 0x9E3779B97F4A7C15ull;   // splitmix64 finaliser
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return static_cast<uint32_t>(z % elems);
+}
+
+__global__ void k_fill_gather_input(int32_t* __restrict__ idx, float* __restrict__ val, int64_t n, uint32_t elems,
+                                    uint64_t seed) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        idx[i] = static_cast<int32_t>(hash_index(static_cast<uint64_t>(i), seed, elems));
+        val[i] = 1.0f;
+    }
+}
+
+template <typename G>
+__global__ __launch_bounds__(kThreads) void k_gather_table(const int32_t* __restrict__ idx,
+                                                           const float* __restrict__ val,
+                                                           const G* __restrict__ tab, G* __restrict__ out) {
+    const int64_t base = int64_t(blockIdx.x) * kTileNnz + threadIdx.x;
+    int j[kPerThread];
+    float a[kPerThread];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        j[q] = stream_load(idx + base + q * kThreads);
+        a[q] = stream_load(val + base + q * kThreads);
+    }
+    G acc = G(0);
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) acc += static_cast<G>(a[q]) * tab[j[q]];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) out[size_t(blockIdx.x) * kWaves + (threadIdx.x >> 6)] = acc;
+}
+
 // Plain row sums out = M v (debug products, power iteration).
 template <typename T>
 struct StoreEpi {

@@ -158,6 +158,15 @@ class Engine:
                     "pdlpx_time_gathers")
         return a.value, b.value
 
+    def gather_rate(self, table_bytes: int, elem_bytes: int = 8, gathers: int = 2048 * 10417,
+                    reps: int = 20) -> float:
+        """Random gathers per second from a table of `table_bytes` (uniform
+        hashed positions, the trial kernels' thread layout)."""
+        r = C.c_double()
+        self._check(self.lib.pdlpx_gather_rate(self.ctx, table_bytes, elem_bytes, gathers, reps, C.byref(r)),
+                    "pdlpx_gather_rate")
+        return r.value
+
     def update_vectors(self, index: int, c=None, rlo=None, rhi=None) -> None:
         """New objective and/or row bounds for loaded instance `index`; the
         matrix stays on the device (pdlpx_update_vectors)."""

@@ -118,3 +118,25 @@ def test_batch_mixed_fp32_values_fp64_iterates(handoff):
         assert abs(r.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-6
         m = ol.evaluate(lp, o, r.primal, r.row_dual)
         assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_compacted_batch_equals_single_solves(precision):
+    """Instances of very different sizes finish at different times, so the
+    trial kernels' plan is cut down to the running instances (and widened
+    again for the final FP64 polish in mixed precision).  Every member must
+    still equal its single solve bitwise."""
+    sizes = [300, 2400, 450, 3000, 600, 1800, 350, 900]
+    lps = [box_lp(m, int(2.5 * m), 5, seed=700 + i) for i, m in enumerate(sizes)]
+    o = _opts(1e-6, precision=precision)
+    print(f"batch of {len(lps)}, precision {precision}", flush=True)
+    with Engine() as e:
+        e.load_batch(lps, precision)
+        res = e.solve_batch(o)
+    for lp, r in zip(lps, res):
+        with Engine() as e1:
+            e1.load(lp, precision)
+            s = e1.solve(o)
+        print(f"  m={lp.m}: {r.status} {r.iterations} its (single {s.iterations})", flush=True)
+        assert (s.status, s.iterations, s.step_trials) == (r.status, r.iterations, r.step_trials)
+        assert np.array_equal(s.primal, r.primal) and np.array_equal(s.row_dual, r.row_dual)

@@ -117,3 +117,16 @@ def test_time_trial_and_gather_floor(sliced):
         r1 = e.solve(o)
     assert r1.iterations == r0.iterations
     assert np.array_equal(r1.primal, r0.primal) and np.array_equal(r1.row_dual, r0.row_dual)
+
+
+def test_gather_rate_probe():
+    """The device gather-rate probe: positive rates, an L1-sized table at
+    least as fast as an HBM-sized one, and bad arguments rejected."""
+    with Engine() as e:
+        small = e.gather_rate(16 * 1024, 8, gathers=2048 * 512, reps=3)
+        big = e.gather_rate(256 << 20, 8, gathers=2048 * 512, reps=3)
+        print(f"gathers/s: 16 KB {small:.3e}, 256 MB {big:.3e}", flush=True)
+        assert small > 0 and big > 0 and small >= big
+        for bad in [(16 * 1024, 3, 4096), (2, 8, 4096), (16 * 1024, 8, 100)]:
+            with pytest.raises(EngineError):
+                e.gather_rate(bad[0], bad[1], gathers=bad[2], reps=1)
