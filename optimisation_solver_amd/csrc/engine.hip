@@ -2378,7 +2378,9 @@ int pdlpx_time_gathers(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms
         const OrientBuf& C = X->sliced ? X->SAT : X->AT;
         DBuf<double> out;
         out.alloc(size_t(std::max(X->A.nblk, C.nblk)) * kWaves);
-        hipEvent_t e[3];
+        // events between the passes, one host wait at the end: the passes run
+        // back to back as the trial kernels do inside a solve
+        std::vector<hipEvent_t> e(size_t(3) * reps, nullptr);
         for (auto& h : e) HIPCHK(hipEventCreate(&h));
         double tr = 0.0, tc = 0.0;
         auto run = [&](auto vtag, auto ttag) {
@@ -2404,15 +2406,17 @@ int pdlpx_time_gathers(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms
             rows();
             cols();
             for (int r = 0; r < reps; ++r) {
-                HIPCHK(hipEventRecord(e[0], s));
+                HIPCHK(hipEventRecord(e[3 * r], s));
                 rows();
-                HIPCHK(hipEventRecord(e[1], s));
+                HIPCHK(hipEventRecord(e[3 * r + 1], s));
                 cols();
-                HIPCHK(hipEventRecord(e[2], s));
-                HIPCHK(hipEventSynchronize(e[2]));
+                HIPCHK(hipEventRecord(e[3 * r + 2], s));
+            }
+            HIPCHK(hipStreamSynchronize(s));
+            for (int r = 0; r < reps; ++r) {
                 float a = 0.f, b = 0.f;
-                HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
-                HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+                HIPCHK(hipEventElapsedTime(&a, e[3 * r], e[3 * r + 1]));
+                HIPCHK(hipEventElapsedTime(&b, e[3 * r + 1], e[3 * r + 2]));
                 tr += a;
                 tc += b;
             }
