@@ -126,8 +126,10 @@ def bench_batch(args) -> int:
     eng.set_progress(lambda run, it, kkt: log(f"[bench]   ... {run} instances running, iteration {it}"), 20.0)
     eng.load_batch(lps, prec)
     tiling = None
-    if args.autotune:
+    if not args.no_autotune:   # the batched workload is specified with autotuned row-block tiles
+        t = time.perf_counter()
         tiling = eng.autotune(prec, args.kernel_reps)
+        tiling["tune_s"] = round(time.perf_counter() - t, 3)
         log(f"[bench] autotune: {tiling}")
     for w in range(args.warmup):
         eng.solve_batch(o)
@@ -321,6 +323,8 @@ def main() -> int:
                     help="PDHG arithmetic: fp64 (reference), fp32 (FP32 values and iterates), mixed (FP32 "
                          "matrix values, FP64 iterates); default mixed for --config 2 and 3, fp64 otherwise")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batched workload (--config 3): keep the default row-block plan instead of autotuning")
     ap.add_argument("--autotune", action="store_true",
                     help="time the candidate SpMV row-block plans after loading and keep the fastest "
                          "(one-time, outside the timed region; reported under config.tiling)")
