@@ -276,6 +276,16 @@ def bench_warmstart(args) -> int:
 
     per_cold = np.median(cold.cold_seconds)
     best = res["warm_mixed"]
+    cpu = None
+    if not args.no_cpu:
+        # the reference, cold, on the first held-out instance: set-up plus a
+        # bounded iteration sample, scaled to the GPU cold median iterations
+        threads = min(16, os.cpu_count() or 1)
+        cold_its = int(np.median(cold.cold_iterations))
+        cpu = cpu_baseline(fam.instance(th[0]), 1e-6, cold_its, args.cpu_sample_iters, threads)
+        cpu["sample"] += (f" (one held-out instance; the GPU runs {B} instances per batch, "
+                          f"{cold.cold_wall / max(nte, 1):.3f} s per instance amortised cold)")
+        log(f"[bench] cpu: {cpu}")
     line = {
         "metric": METRIC, "value": round(float(np.median(best.warm_seconds)), 4), "unit": "s",
         "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round(best.warm_wall / max(nte, 1) * 1e3, 3),
@@ -296,7 +306,7 @@ def bench_warmstart(args) -> int:
         "roofline": {"bound": "mfma", "achieved": round(mlp_tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(mlp_tflops / 2500.0, 5), "traffic": None, "kernel": "k_gemm (bf16 MFMA)",
                      "note": "network training GEMMs (batch 256, 256-512 wide): latency-bound by size"},
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
     return 0
