@@ -276,6 +276,16 @@ def bench_warmstart(args) -> int:
 
     per_cold = np.median(cold.cold_seconds)
     best = res["warm_mixed"]
+    # one held-out instance alone on the device (latency rather than batch throughput)
+    from optimisation_solver_amd import Engine, default_options
+    single = Engine(0)
+    single.load(fam.instance(th[0]))
+    o1 = default_options(primal_tolerance=1e-6, dual_tolerance=1e-6, gap_tolerance=1e-6)
+    t = time.perf_counter()
+    r1 = single.solve(o1)
+    single_s = time.perf_counter() - t
+    single.close()
+    log(f"[bench] one held-out instance alone, cold FP64: {r1.status} {r1.iterations} its in {single_s:.3f}s")
     cpu = None
     if not args.no_cpu:
         # the reference, cold, on the first held-out instance: set-up plus a
@@ -296,7 +306,9 @@ def bench_warmstart(args) -> int:
                    "tolerance": 1e-6, "device_batch": B,
                    "cold_fp64": {"iterations_p50": float(np.median(cold.cold_iterations)),
                                  "per_instance_s_p50": round(float(per_cold), 4), "wall_s": round(cold.cold_wall, 3),
-                                 "worst_kkt": float(np.max(cold.cold_kkt))},
+                                 "worst_kkt": float(np.max(cold.cold_kkt)),
+                                 "single_instance_s": round(single_s, 4),
+                                 "single_instance_iterations": int(r1.iterations)},
                    "warm": {k: summary(v) for k, v in res.items()},
                    "network": {"dims": [fam.dim, 512, 512, 512], "best_val_loss": fit["best_val_loss"],
                                "val_loss_by_round": fit["val_loss"], "explained_x": fit["explained_x"],
