@@ -44,6 +44,8 @@ def cases():
     for name, (lp, *_rest) in problems.closed_form().items():
         tol = 1e-8 if name == "ill_conditioned" else 2e-6
         out.append((name, lp, opts(tol, **strict)))
+    for name, (lp, _obj) in problems.structural().items():
+        out.append((name, lp, opts(1e-8, **strict)))
     lim = dict(strict, iteration_limit=20000)
     out.append(("infeasible_rows", problems.infeasible_rows(), opts(2e-6, **lim)))
     out.append(("infeasible_equalities", problems.infeasible_equalities(), opts(2e-6, **lim)))

@@ -52,6 +52,24 @@ def closed_form():
     return out
 
 
+# name -> (lp, expected objective): structural corner cases of the LP format
+# (compiled_lp.h: empty rows and columns, fixed variables, no rows, free rows)
+def structural():
+    return {
+        # row 1 has no entries and 0 lies in its range; column 2 has no entries,
+        # so it sits at its cheaper bound: min x0 + 2 x1 + 2 x2 -> (1, 0, -3)
+        "empty_row_and_column": (lp(2, 3, [(0, 0, 1.0), (0, 1, 1.0)], [1, 2, 2], [0, 0, -3], [10, 10, 4],
+                                    [1, -1], [INF, 5]), -5.0),
+        # x1 fixed at 2 by lo == hi: min x0 + x1, x0 + x1 >= 3 -> (1, 2)
+        "fixed_variable": (lp(1, 2, [(0, 0, 1.0), (0, 1, 1.0)], [1, 1], [0, 2], [10, 2], [3], [INF]), 3.0),
+        # no rows at all: the box alone, min x0 - x1 over [0, 1]^2 -> (0, 1)
+        "no_rows": (lp(0, 2, [], [1, -1], [0, 0], [1, 1], [], []), -1.0),
+        # row 1 is free (-inf, inf) and never binds: min x0 + x1, x0 + x1 >= 1
+        "free_row": (lp(2, 2, [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, -1.0)], [1, 1], [0, 0], [5, 5],
+                        [1, -INF], [INF, INF]), 1.0),
+    }
+
+
 def infeasible_rows():
     # test_pdlp.cpp:608-640  x + y >= 2 and x + y <= 1
     return lp(2, 2, [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 1.0)], [1, 1], [0, 0], [10, 10],

@@ -104,6 +104,26 @@ def test_closed_form_solves(name):
         assert gpu.row_dual[1] == 0.0
 
 
+@pytest.mark.parametrize("name", list(problems.structural()))
+def test_structural_edge_cases(name):
+    """Empty rows and columns, fixed variables, an LP with no rows and a free
+    row: the oracle's status, the optimum fixed by inspection, and the oracle's
+    KKT bar on the GPU point."""
+    lp, obj = problems.structural()[name]
+    tol = 1e-8
+    o = _opts(tol, iteration_limit=100000, termination_check_frequency=50, polishing_iterations=2000)
+    print(f"structural {name}", flush=True)
+    with Engine() as e:
+        e.load(lp)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, o)
+    assert gpu.status == cpu.status_name == "optimal"
+    assert abs(gpu.primal_objective - obj) <= 1e-7 * (1 + abs(obj))
+    assert abs(gpu.primal_objective - cpu.primal_objective) <= 1e-6 * (1 + abs(cpu.primal_objective))
+    m = ol.evaluate(lp, o, gpu.primal, gpu.row_dual)
+    assert m["primal_residual"] <= tol and m["dual_residual"] <= tol and m["relative_gap"] <= tol
+
+
 def test_c0_matches_oracle():
     lp = box_lp(2000, 5000, 5)
     o = _opts(1e-4)

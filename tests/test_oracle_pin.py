@@ -188,6 +188,17 @@ def test_oracle_closed_form_optima(name):
         assert np.allclose(r.x, x, atol=2e-4)
 
 
+@pytest.mark.parametrize("name", list(problems.structural()))
+def test_oracle_structural_optima(name):
+    """Empty rows and columns, fixed variables, no rows, free rows: the
+    oracle reaches the optimum fixed by inspection."""
+    lp, obj = problems.structural()[name]
+    o = opts(1e-8, iteration_limit=100000, termination_check_frequency=50, polishing_iterations=2000)
+    r = ol.solve(lp, o)
+    assert r.status_name == "optimal"
+    assert abs(r.primal_objective - obj) <= 1e-7 * (1 + abs(obj))
+
+
 def test_step_controller_table():
     o = opts(1e-6)
     eta_out = ol.C.c_double()
