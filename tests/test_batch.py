@@ -140,3 +140,27 @@ def test_compacted_batch_equals_single_solves(precision):
         print(f"  m={lp.m}: {r.status} {r.iterations} its (single {s.iterations})", flush=True)
         assert (s.status, s.iterations, s.step_trials) == (r.status, r.iterations, r.step_trials)
         assert np.array_equal(s.primal, r.primal) and np.array_equal(s.row_dual, r.row_dual)
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_batch_with_structural_corner_cases(precision):
+    """Instances with an empty row and column, a fixed variable, no rows at
+    all and a free row, batched with ordinary LPs: every member equals its
+    single solve bitwise and reaches the optimum fixed by inspection."""
+    st = problems.structural()
+    lps = [box_lp(400, 1000, 5, seed=81)] + [v[0] for v in st.values()] + [box_lp(900, 2250, 5, seed=82)]
+    o = _opts(1e-8, precision=precision, iteration_limit=100000, termination_check_frequency=50,
+              polishing_iterations=2000)
+    print(f"batch of {len(lps)} with corner cases, precision {precision}", flush=True)
+    with Engine() as e:
+        e.load_batch(lps, precision)
+        res = e.solve_batch(o)
+    for lp, r in zip(lps, res):
+        with Engine() as e1:
+            e1.load(lp, precision)
+            s = e1.solve(o)
+        print(f"  m={lp.m} n={lp.n}: {r.status} {r.iterations} its (single {s.iterations})", flush=True)
+        assert (s.status, s.iterations, s.step_trials) == (r.status, r.iterations, r.step_trials)
+        assert np.array_equal(s.primal, r.primal) and np.array_equal(s.row_dual, r.row_dual)
+    for (lp, obj), r in zip(st.values(), res[1:-1]):
+        assert r.status == "optimal" and abs(r.primal_objective - obj) <= 1e-6 * (1 + abs(obj))
