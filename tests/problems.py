@@ -70,6 +70,25 @@ def structural():
     }
 
 
+# name -> option overrides: solver variants the reference exposes in
+# PdlpOptions (pdlp_options.h), exercised on random_feasible(300, 420, 21)
+def option_variants():
+    return {
+        "no_ruiz": dict(use_ruiz_scaling=0),
+        "ruiz_3": dict(ruiz_iterations=3),
+        "no_averaging": dict(use_averaging=0),
+        "no_restarts": dict(use_restarts=0),
+        "no_pc_no_ruiz": dict(use_diagonal_preconditioning=0, use_ruiz_scaling=0),
+        "fixed_step_no_averaging": dict(use_adaptive_linesearch=0, use_averaging=0),
+        "weight_smoothing_0": dict(primal_weight_smoothing=0.0),
+        "initial_weight_0.1": dict(initial_primal_weight=0.1),
+        "restart_window": dict(minimum_restart_iterations=10, maximum_restart_iterations=500),
+        "check_every_7": dict(termination_check_frequency=7),
+        "step_safety_0.5": dict(initial_step_safety=0.5),
+        "one_step_trial": dict(maximum_step_trials=1),
+    }
+
+
 def infeasible_rows():
     # test_pdlp.cpp:608-640  x + y >= 2 and x + y <= 1
     return lp(2, 2, [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 1.0)], [1, 1], [0, 0], [10, 10],

@@ -124,6 +124,24 @@ def test_structural_edge_cases(name):
     assert m["primal_residual"] <= tol and m["dual_residual"] <= tol and m["relative_gap"] <= tol
 
 
+@pytest.mark.parametrize("name", list(problems.option_variants()))
+def test_option_variants_match_oracle(name):
+    """Each PdlpOptions variant on the device: the oracle's status, and for an
+    optimal solve the objective within 1e-6 and the oracle's KKT bar."""
+    lp = problems.random_feasible(300, 420, 21)
+    tol = 1e-6
+    o = _opts(tol, iteration_limit=100000, use_feasibility_polishing=0, **problems.option_variants()[name])
+    print(f"option variant {name}", flush=True)
+    with Engine() as e:
+        e.load(lp)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, o)
+    print(f"  gpu {gpu.status} {gpu.iterations} its; cpu {cpu.status_name} {cpu.iterations} its", flush=True)
+    assert gpu.status == cpu.status_name
+    if cpu.status_name == "optimal":
+        _check_solution(lp, o, gpu, cpu, tol)
+
+
 def test_c0_matches_oracle():
     lp = box_lp(2000, 5000, 5)
     o = _opts(1e-4)

@@ -199,6 +199,21 @@ def test_oracle_structural_optima(name):
     assert abs(r.primal_objective - obj) <= 1e-7 * (1 + abs(obj))
 
 
+@needs_ref
+@pytest.mark.parametrize("name", list(problems.option_variants()))
+def test_oracle_bitwise_vs_reference_option_variants(name):
+    """Every PdlpOptions switch the reference exposes (scaling, averaging,
+    restarts, linesearch, primal weight, check frequency, trial budget): the
+    oracle's trajectory equals the compiled reference's bit for bit."""
+    lp = problems.random_feasible(300, 420, 21)
+    o = opts(1e-6, iteration_limit=100000, use_feasibility_polishing=0, **problems.option_variants()[name])
+    a = ol.solve(lp, o, "oracle")
+    b = ol.solve(lp, o, "ref")
+    for k in KEYS:
+        assert same(a.d[k], b.d[k]), (name, k, a.d[k], b.d[k])
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+
+
 def test_step_controller_table():
     o = opts(1e-6)
     eta_out = ol.C.c_double()
