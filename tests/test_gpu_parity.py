@@ -142,6 +142,32 @@ def test_option_variants_match_oracle(name):
         _check_solution(lp, o, gpu, cpu, tol)
 
 
+def test_iteration_and_time_limits():
+    """The limit statuses and messages of pdlp_solver.cpp: an iteration limit
+    stops at the oracle's iteration count with the oracle's status and
+    message; a time limit stops a long solve with "Time limit reached" and a
+    finite point."""
+    lp = box_lp(2000, 5000, 5)
+    o = _opts(1e-9, iteration_limit=777)
+    with Engine() as e:
+        e.load(lp)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, o)
+    print(f"iteration limit: gpu {gpu.status} {gpu.iterations} '{gpu.status_message}'; "
+          f"cpu {cpu.status_name} {cpu.iterations}", flush=True)
+    assert gpu.status == cpu.status_name == "iteration_limit"
+    assert gpu.iterations == cpu.iterations
+    assert gpu.status_message == "Iteration limit reached"
+    big = zipf_lp(20000, 50000, seed=11)
+    o = _opts(1e-12, time_limit_seconds=0.05)
+    with Engine() as e:
+        e.load(big)
+        gpu = e.solve(o)
+    print(f"time limit: gpu {gpu.status} {gpu.iterations} its in {gpu.solve_time_seconds:.3f}s", flush=True)
+    assert gpu.status == "time_limit" and gpu.status_message == "Time limit reached"
+    assert gpu.iterations > 0 and np.all(np.isfinite(gpu.primal)) and np.all(np.isfinite(gpu.row_dual))
+
+
 def test_c0_matches_oracle():
     lp = box_lp(2000, 5000, 5)
     o = _opts(1e-4)
