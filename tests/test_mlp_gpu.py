@@ -28,8 +28,10 @@ U = 2.0 ** -8
 
 @pytest.mark.parametrize("ta", [False, True])
 @pytest.mark.parametrize("tb", [False, True])
-@pytest.mark.parametrize("shape", [(64, 64, 32), (128, 192, 96), (256, 512, 512)])
+@pytest.mark.parametrize("shape", [(64, 64, 32), (128, 192, 96), (64, 128, 224), (128, 64, 256), (192, 64, 480),
+                                   (256, 512, 512)])
 def test_gemm_layouts(shape, ta, tb):
+    """Every layout, K within one LDS chunk (224) and across chunk boundaries."""
     M, N, K = shape
     g = np.random.default_rng(M + N + K)
     A = g.standard_normal((K, M) if ta else (M, K)).astype(np.float32)
