@@ -2,12 +2,11 @@
 // FP32 master weights and Adam.  The arithmetic contract is spelled out in
 // include/pdlpx_mlp.h; oracle/mlp_oracle.py restates it in FP64 on the CPU.
 //
-// GEMM tiling (wave64): one 256-thread workgroup per 64 x 64 output tile.
-// K is staged in chunks of up to 224: the A tile (64 rows x chunk) and the B
-// tile (64 columns x chunk) go to LDS, both k-contiguous, with every global
-// load of the chunk in flight at once; MFMA steps of 32 k then run over the
-// chunk.  Every lane reads its MFMA operand (8 consecutive k of one row or
-// column) as one 16-byte LDS load.  Wave w owns the 32 x 32 quadrant (w / 2, w % 2) of the tile as 2 x 2
+// GEMM tiling (wave64): one 256-thread workgroup per 64 x 64 output tile,
+// K in steps of 32.  Each step stages the A tile (64 rows x 32 k) and the B
+// tile (64 columns x 32 k) in LDS, both k-contiguous, so every lane reads its
+// MFMA operand (8 consecutive k of one row or column) as one 16-byte LDS
+// load.  Wave w owns the 32 x 32 quadrant (w / 2, w % 2) of the tile as 2 x 2
 // v_mfma_f32_16x16x32_bf16 blocks accumulating in FP32 registers.  Operand
 // layout of that instruction: lane l supplies row (A) / column (B) l % 16 and
 // k = 8 (l / 16) .. 8 (l / 16) + 7; it returns column l % 16, rows
@@ -57,10 +56,8 @@ __host__ __device__ inline bf16 to_bf16(float f) {
 __host__ __device__ inline float from_bf16(bf16 h) { return __builtin_bit_cast(float, uint32_t(h) << 16); }
 
 constexpr int kBM = 64, kBN = 64, kBK = 32;
-constexpr int kKC = 224;         // K staged per round: 7 MFMA steps of 32
-constexpr int kLd = kKC + 8;     // LDS row stride in bf16 (464 bytes): 16 rows hit 64 distinct banks
+constexpr int kLd = kBK + 8;   // LDS row stride in bf16 (80 bytes: 16-byte aligned rows)
 constexpr int kGemmThreads = 256;
-constexpr int kChunkLoads = kBM * kKC / 8 / kGemmThreads;   // 16-byte loads per thread per operand (7)
 
 struct Epi {
     const float* bias;   // [N] or null
@@ -72,56 +69,16 @@ struct Epi {
     int ldc;
 };
 
-// One operand's tile rows [r0, r0 + 64) x k [k0, k0 + kc), rows k-contiguous
-// in LDS.  T: the operand is stored k-major (element (r, k) at P[k ld + r])
-// and is transposed on the way in.  load_chunk issues all of a thread's
-// 16-byte loads; store_chunk writes them to LDS, so the two operands' loads
-// of a chunk are in flight together and a chunk costs one global round trip.
-template <int T>
-__device__ inline void load_chunk(const bf16* __restrict__ P, int ld, int r0, int k0, int kc,
-                                  uint4 (&v)[kChunkLoads]) {
-    const int per = T ? 8 : kc / 8;   // 16-byte pieces per stored line (a row, or a k line when T)
-#pragma unroll
-    for (int u = 0; u < kChunkLoads; ++u) {
-        const int q = static_cast<int>(threadIdx.x) + u * kGemmThreads;
-        const int line = q / per, ch = q - line * per;
-        const bool ok = T ? line < kc : line < kBM;
-        v[u] = ok ? *reinterpret_cast<const uint4*>(T ? P + size_t(k0 + line) * ld + r0 + 8 * ch
-                                                      : P + size_t(r0 + line) * ld + k0 + 8 * ch)
-                  : uint4{0u, 0u, 0u, 0u};
-    }
-}
-
-template <int T>
-__device__ inline void store_chunk(const uint4 (&v)[kChunkLoads], int kc, bf16* S) {
-    const int per = T ? 8 : kc / 8;
-#pragma unroll
-    for (int u = 0; u < kChunkLoads; ++u) {
-        const int q = static_cast<int>(threadIdx.x) + u * kGemmThreads;
-        const int line = q / per, ch = q - line * per;
-        if (T ? line < kc : line < kBM) {
-            if (!T) {
-                *reinterpret_cast<uint4*>(&S[line * kLd + 8 * ch]) = v[u];
-            } else {
-                const bf16* el = reinterpret_cast<const bf16*>(&v[u]);
-#pragma unroll
-                for (int e = 0; e < 8; ++e) S[(8 * ch + e) * kLd + line] = el[e];
-            }
-        }
-    }
-}
-
 // C(m, n) = sum_k A(m, k) B(k, n);  A(m, k) = TA ? A[k lda + m] : A[m lda + k],
-// B(k, n) = TB ? B[n ldb + k] : B[k ldb + n].  Grid (N / 64, M / 64).  K runs
-// in chunks of up to kKC, each staged whole in LDS (one global round trip)
-// and consumed by kKC / 32 MFMA steps with no barrier between them.
+// B(k, n) = TB ? B[n ldb + k] : B[k ldb + n].  Grid (N / 64, M / 64).
 template <int TA, int TB>
 __global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16* __restrict__ A, int lda,
                                                        const bf16* __restrict__ B, int ldb, int K, Epi e) {
     __shared__ __attribute__((aligned(16))) bf16 As[kBM * kLd];
     __shared__ __attribute__((aligned(16))) bf16 Bs[kBN * kLd];
+    const int t = threadIdx.x;
     const int m0 = blockIdx.y * kBM, n0 = blockIdx.x * kBN;
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int w = t >> 6, l = t & 63;
     const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
     f32x4 acc[2][2];
 #pragma unroll
@@ -129,29 +86,42 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm(const bf16* __restrict__ 
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < K; k0 += kKC) {
-        const int kc = min(kKC, K - k0);   // a multiple of 32
-        uint4 va[kChunkLoads], vb[kChunkLoads];
-        load_chunk<TA>(A, lda, m0, k0, kc, va);
-        load_chunk<!TB>(B, ldb, n0, k0, kc, vb);
-        store_chunk<TA>(va, kc, As);
-        store_chunk<!TB>(vb, kc, Bs);
-        __syncthreads();
-        for (int kk = 0; kk < kc; kk += kBK) {
-            bf16x8 a[2], b[2];
+    for (int k0 = 0; k0 < K; k0 += kBK) {
+        if (!TA) {   // 64 rows x 4 chunks of 8 k
+            const int r = t >> 2, ch = t & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(A + size_t(m0 + r) * lda + k0 + 8 * ch);
+            *reinterpret_cast<uint4*>(&As[r * kLd + 8 * ch]) = v;
+        } else {     // 32 k rows x 8 chunks of 8 m, transposed into LDS
+            const int kk = t >> 3, ch = t & 7;
+            const uint4 v = *reinterpret_cast<const uint4*>(A + size_t(k0 + kk) * lda + m0 + 8 * ch);
+            const bf16* el = reinterpret_cast<const bf16*>(&v);
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                a[i] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(&As[(wm + 16 * i + (l & 15)) * kLd + kk + 8 * (l >> 4)]));
-                b[i] = __builtin_bit_cast(
-                    bf16x8, *reinterpret_cast<const uint4*>(&Bs[(wn + 16 * i + (l & 15)) * kLd + kk + 8 * (l >> 4)]));
-            }
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int q = 0; q < 8; ++q) As[(8 * ch + q) * kLd + kk] = el[q];
         }
+        if (TB) {
+            const int r = t >> 2, ch = t & 3;
+            const uint4 v = *reinterpret_cast<const uint4*>(B + size_t(n0 + r) * ldb + k0 + 8 * ch);
+            *reinterpret_cast<uint4*>(&Bs[r * kLd + 8 * ch]) = v;
+        } else {
+            const int kk = t >> 3, ch = t & 7;
+            const uint4 v = *reinterpret_cast<const uint4*>(B + size_t(k0 + kk) * ldb + n0 + 8 * ch);
+            const bf16* el = reinterpret_cast<const bf16*>(&v);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) Bs[(8 * ch + q) * kLd + kk] = el[q];
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            a[i] = __builtin_bit_cast(bf16x8,
+                                      *reinterpret_cast<const uint4*>(&As[(wm + 16 * i + (l & 15)) * kLd + 8 * (l >> 4)]));
+            b[i] = __builtin_bit_cast(bf16x8,
+                                      *reinterpret_cast<const uint4*>(&Bs[(wn + 16 * i + (l & 15)) * kLd + 8 * (l >> 4)]));
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
 #pragma unroll
