@@ -142,6 +142,28 @@ def test_option_variants_match_oracle(name):
         _check_solution(lp, o, gpu, cpu, tol)
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("name", [n for n in problems.option_variants() if n != "one_step_trial"])
+def test_option_variants_reduced_precision(name, precision):
+    """The option variants with FP32 values (and FP32 or FP64 iterates): the
+    oracle's status, and the GPU point meets the oracle's KKT bar at 1e-6."""
+    lp = problems.random_feasible(300, 420, 21)
+    tol = 1e-6
+    o = _opts(tol, iteration_limit=200000, use_feasibility_polishing=0, precision=precision,
+              **problems.option_variants()[name])
+    print(f"option variant {name}, precision {precision}", flush=True)
+    with Engine() as e:
+        e.load(lp, precision)
+        gpu = e.solve(o)
+    cpu = ol.solve(lp, _opts(tol, iteration_limit=200000, use_feasibility_polishing=0,
+                             **problems.option_variants()[name]))
+    print(f"  gpu {gpu.status} {gpu.iterations} its; cpu {cpu.status_name} {cpu.iterations} its", flush=True)
+    assert gpu.status == cpu.status_name == "optimal"
+    m = ol.evaluate(lp, o, gpu.primal, gpu.row_dual)
+    assert m["primal_residual"] <= tol and m["dual_residual"] <= tol and m["relative_gap"] <= tol
+    assert abs(gpu.primal_objective - cpu.primal_objective) / (1 + abs(cpu.primal_objective)) <= 1e-4
+
+
 def test_iteration_and_time_limits():
     """The limit statuses and messages of pdlp_solver.cpp: an iteration limit
     stops at the oracle's iteration count with the oracle's status and
