@@ -238,6 +238,33 @@ def test_infeasible_and_unbounded_statuses():
         assert e.solve(o2).status != "optimal"
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+def test_infeasible_and_unbounded_reduced_precision(precision):
+    """Infeasibility and unboundedness certificates with FP32 values: the ray
+    tests run on the original FP64 matrix, so the statuses are the oracle's
+    and the dual ray certifies the infeasible rows."""
+    o = _opts(2e-6, iteration_limit=20000, termination_check_frequency=50, polishing_iterations=2000,
+              precision=precision)
+    oc = _opts(2e-6, iteration_limit=20000, termination_check_frequency=50, polishing_iterations=2000)
+    for q, (lp, want) in enumerate(((problems.infeasible_rows(), "infeasible"),
+                                     (problems.infeasible_equalities(), "infeasible"),
+                                     (problems.unbounded(), "unbounded"))):
+        with Engine() as e:
+            e.load(lp, precision)
+            r = e.solve(o)
+        cpu = ol.solve(lp, oc)
+        print(f"precision {precision}: gpu {r.status} {r.iterations} its; cpu {cpu.status_name}", flush=True)
+        assert r.status == cpu.status_name == want
+        if q == 0:   # x + y >= 2 and x + y <= 1 over [0, 10]^2: the ray separates
+            y = r.dual_ray
+            red = y[0] + y[1]
+            box = (0.0 if red > 0 else red * 10.0) * 2
+            support = (0.0 if y[0] > 0 else y[0] * 2.0) + (y[1] * 1.0 if y[1] > 0 else 0.0)
+            assert box - support > 1e-9
+        if want == "unbounded":
+            assert r.primal_ray[0] > 0
+
+
 def test_invalid_problem_and_options():
     lp = problems.lp(1, 1, [(0, 0, 1.0)], [1], [0], [1], [5], [1])   # crossed rows
     with Engine() as e:
