@@ -71,3 +71,26 @@ def test_gpu_engine_known_answers(name):
         assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-8
     else:
         assert r.status in ("infeasible", "unbounded")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(KNOWN["optimal"]) + KNOWN["infeasible"] + KNOWN["unbounded"])
+def test_gpu_engine_known_answers_mixed_precision(name):
+    """FP32 matrix values with FP64 iterates at 1e-6: the recorded status, and
+    the recorded optimum to 1e-5 relative with the oracle's KKT bar met."""
+    model = read_mps(str(GOLD / name))
+    o = default_options(primal_tolerance=1e-6, dual_tolerance=1e-6, gap_tolerance=1e-6, thread_count=1,
+                        iteration_limit=200000, precision=2)
+    with Engine() as e:
+        e.load(model.lp, 2)
+        r = e.solve(o)
+    print(f"{name}: gpu mixed {r.status} {r.iterations} its objective {model.objective(r.primal_objective):.10g}",
+          flush=True)
+    if name in KNOWN["optimal"]:
+        want = KNOWN["optimal"][name]
+        assert r.status == "optimal"
+        assert abs(model.objective(r.primal_objective) - want) <= 1e-5 * (1 + abs(want))
+        m = ol.evaluate(model.lp, o, r.primal, r.row_dual)
+        assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= 1e-6
+    else:
+        assert r.status == ("infeasible" if name in KNOWN["infeasible"] else "unbounded")
