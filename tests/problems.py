@@ -89,6 +89,20 @@ def option_variants():
     }
 
 
+# name -> LP that CompiledLp::validate rejects (compiled_lp.h:32-62)
+def invalid():
+    nan = math.nan
+    one = [(0, 0, 1.0)]
+    return {
+        "nan_cost": lp(1, 1, one, [nan], [0], [1], [0], [1]),
+        "crossed_var_bounds": lp(1, 1, one, [1], [2], [1], [0], [1]),
+        "nan_var_bound": lp(1, 1, one, [1], [nan], [1], [0], [1]),
+        "infinite_offset": lp(1, 1, one, [1], [0], [1], [0], [1], offset=INF),
+        "crossed_rows": lp(1, 1, one, [1], [0], [1], [5], [1]),
+        "nan_row_bound": lp(1, 1, one, [1], [0], [1], [nan], [1]),
+    }
+
+
 def infeasible_rows():
     # test_pdlp.cpp:608-640  x + y >= 2 and x + y <= 1
     return lp(2, 2, [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 1.0)], [1, 1], [0, 0], [10, 10],

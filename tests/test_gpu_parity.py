@@ -278,6 +278,25 @@ def test_invalid_problem_and_options():
     assert r.status == "invalid_problem" and r.status_message == "Invalid PDLP options"
 
 
+@pytest.mark.parametrize("name", list(problems.invalid()))
+def test_invalid_problems_through_the_solver_facade(name):
+    """PdlpSolver.solve never raises for a rejected LP: it returns the
+    oracle's invalid_problem status and message, alone and inside a batch
+    (where the other member still solves)."""
+    from optimisation_solver_amd.solver import PdlpSolver
+    lp = problems.invalid()[name]
+    cpu = ol.solve(lp, _opts(1e-6))
+    r = PdlpSolver().solve(lp, _opts(1e-6))
+    print(f"{name}: gpu {r.status} '{r.status_message}'; cpu {cpu.status_name} '{cpu.d['message']}'", flush=True)
+    assert (r.status, r.status_message) == (cpu.status_name, cpu.d["message"])
+    good = problems.closed_form()["equality_row"][0]
+    with Engine() as e:
+        e.load_batch([lp, good])
+        res = e.solve_batch(_opts(1e-6))
+    assert (res[0].status, res[0].status_message) == (cpu.status_name, cpu.d["message"])
+    assert res[1].status == "optimal"
+
+
 def test_graph_replay_equals_eager_launch():
     """The captured hipGraph replays the same kernels in the same order as the
     eager launch path: every replay (many per solve, with host work between

@@ -214,6 +214,18 @@ def test_oracle_bitwise_vs_reference_option_variants(name):
     assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
 
 
+@pytest.mark.parametrize("name", list(problems.invalid()))
+def test_oracle_invalid_problem_messages(name):
+    """Rejected LPs: the oracle's status and message equal the compiled
+    reference's (and are pinned by text)."""
+    lp = problems.invalid()[name]
+    a = ol.solve(lp, opts(1e-6))
+    assert a.status_name == "invalid_problem"
+    if ol.have_ref():
+        b = ol.solve(lp, opts(1e-6), "ref")
+        assert (a.status_name, a.d["message"]) == (b.status_name, b.d["message"])
+
+
 def test_step_controller_table():
     o = opts(1e-6)
     eta_out = ol.C.c_double()
