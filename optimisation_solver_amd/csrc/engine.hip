@@ -28,6 +28,7 @@
 #include <memory>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace pdlpx_dev;
@@ -73,6 +74,21 @@ struct DBuf {
         if (count) HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
 };
+
+// Runs fn(begin, end) over [0, n) split into contiguous ranges on up to 16
+// host threads (one when the work is small).
+template <typename Fn>
+void parallelRanges(int64_t n, int64_t work, Fn fn) {
+    const int hw = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+    const int T = static_cast<int>(std::min<int64_t>({16, hw, std::max<int64_t>(1, work >> 20), std::max<int64_t>(1, n)}));
+    if (T <= 1) {
+        fn(int64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < T; ++t) pool.emplace_back(fn, n * t / T, n * (t + 1) / T);
+    for (auto& th : pool) th.join();
+}
 
 int gridFor(int64_t n, int threads = kThreads) {
     return static_cast<int>(std::max<int64_t>(1, (n + threads - 1) / threads));
@@ -1729,17 +1745,34 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     // rows use sit next to each other: their gathers (x' in A x', y in A^T y)
     // then share cache lines instead of touching one line each.  Smaller
     // batches keep the plain degree order.
-    std::vector<int32_t> order;
+    // Order = (degree descending, tie key ascending, index ascending), the
+    // order a stable sort on (degree, key) gives, built from two stable
+    // counting sorts: by key, then by degree.
+    std::vector<int32_t> order, tmp, bucket;
     std::vector<int32_t> key;
-    auto sortSpace = [&](int count, int off, const int64_t* start, const int32_t* tie, std::vector<int32_t>& nw,
-                         std::vector<int32_t>& old) {
+    auto sortSpace = [&](int count, int off, const int64_t* start, const int32_t* tie, int keyRange,
+                         std::vector<int32_t>& nw, std::vector<int32_t>& old) {
         order.resize(count);
         for (int j = 0; j < count; ++j) order[j] = j;
-        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-            const int64_t da = start[a + 1] - start[a], db = start[b + 1] - start[b];
-            if (da != db) return da > db;
-            return tie && tie[a] < tie[b];
-        });
+        if (tie) {   // keys in [0, keyRange) or INT32_MAX (no entry), the latter last
+            bucket.assign(size_t(keyRange) + 2, 0);
+            for (int j = 0; j < count; ++j) ++bucket[tie[j] == INT32_MAX ? keyRange + 1 : tie[j] + 1];
+            for (int b = 0; b <= keyRange; ++b) bucket[b + 1] += bucket[b];
+            tmp.resize(count);
+            for (int j = 0; j < count; ++j) tmp[bucket[tie[j] == INT32_MAX ? keyRange : tie[j]]++] = j;
+            order.swap(tmp);
+        }
+        int64_t maxDeg = 0;
+        for (int j = 0; j < count; ++j) maxDeg = std::max(maxDeg, start[j + 1] - start[j]);
+        bucket.assign(size_t(maxDeg) + 2, 0);   // bucket b holds degree maxDeg - b
+        for (int j = 0; j < count; ++j) ++bucket[maxDeg - (start[j + 1] - start[j]) + 1];
+        for (int64_t b = 0; b <= maxDeg; ++b) bucket[b + 1] += bucket[b];
+        tmp.resize(count);
+        for (int q = 0; q < count; ++q) {
+            const int32_t j = order[q];
+            tmp[bucket[maxDeg - (start[j + 1] - start[j])]++] = j;
+        }
+        order.swap(tmp);
         for (int q = 0; q < count; ++q) {
             old[off + q] = off + order[q];
             nw[off + order[q]] = off + q;
@@ -1747,7 +1780,7 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
     };
     for (int k = 0; k < B; ++k) {
         const pdlpx_problem& p = probs[X.dev[k]];
-        sortSpace(p.rows, X.rowOff[k], p.csr_row_start, nullptr, X.yNew, X.yOld);
+        sortSpace(p.rows, X.rowOff[k], p.csr_row_start, nullptr, 0, X.yNew, X.yOld);
         const bool byRow = X.n > 262144;
         if (byRow) {
             key.assign(size_t(p.cols), INT32_MAX);   // first row of each column in the sorted row order
@@ -1755,43 +1788,68 @@ void loadBatch(pdlpx_ctx& X, int count, const pdlpx_problem* probs, int32_t prec
                 for (int64_t e = p.csc_col_start[j]; e < p.csc_col_start[j + 1]; ++e)
                     key[j] = std::min(key[j], X.yNew[X.rowOff[k] + p.csc_row[e]]);
         }
-        sortSpace(p.cols, X.colOff[k], p.csc_col_start, byRow ? key.data() : nullptr, X.xNew, X.xOld);
+        sortSpace(p.cols, X.colOff[k], p.csc_col_start, byRow ? key.data() : nullptr, X.m, X.xNew, X.xOld);
     }
     std::vector<int32_t> rp(X.m + 1), cp(X.n + 1), ci(total), ri(total);
     std::vector<double> rv(total), cv(total), c(X.n), lo(X.n), hi(X.n), rlo(X.m), rhi(X.m);
+    // Row (column) starts in internal order, then every row (column) copied
+    // in parallel: the copies are independent once the starts are known.
+    std::vector<int32_t> instOfRow(X.m), instOfCol(X.n);
     int64_t z = 0;
-    for (int k = 0; k < B; ++k) {   // CSR rows in internal order
+    for (int k = 0; k < B; ++k) {
         const pdlpx_problem& p = probs[X.dev[k]];
-        const int R = X.rowOff[k], Cc = X.colOff[k];
+        const int R = X.rowOff[k];
         for (int q = 0; q < p.rows; ++q) {
             const int i = X.yOld[R + q] - R;
             rp[R + q] = static_cast<int32_t>(z);
-            for (int64_t e = p.csr_row_start[i]; e < p.csr_row_start[i + 1]; ++e, ++z) {
-                ci[z] = X.xNew[Cc + p.csr_col[e]];
-                rv[z] = p.csr_val[e];
-            }
-            rlo[R + q] = p.row_lower[i];
-            rhi[R + q] = p.row_upper[i];
+            z += p.csr_row_start[i + 1] - p.csr_row_start[i];
+            instOfRow[R + q] = k;
         }
     }
     rp[X.m] = static_cast<int32_t>(z);
     z = 0;
-    for (int k = 0; k < B; ++k) {   // CSC columns in internal order
+    for (int k = 0; k < B; ++k) {
         const pdlpx_problem& p = probs[X.dev[k]];
-        const int R = X.rowOff[k], Cc = X.colOff[k];
+        const int Cc = X.colOff[k];
         for (int q = 0; q < p.cols; ++q) {
             const int j = X.xOld[Cc + q] - Cc;
             cp[Cc + q] = static_cast<int32_t>(z);
-            for (int64_t e = p.csc_col_start[j]; e < p.csc_col_start[j + 1]; ++e, ++z) {
-                ri[z] = X.yNew[R + p.csc_row[e]];
-                cv[z] = p.csc_val[e];
-            }
-            c[Cc + q] = p.objective[j];
-            lo[Cc + q] = p.var_lower[j];
-            hi[Cc + q] = p.var_upper[j];
+            z += p.csc_col_start[j + 1] - p.csc_col_start[j];
+            instOfCol[Cc + q] = k;
         }
     }
     cp[X.n] = static_cast<int32_t>(z);
+    parallelRanges(X.m, total, [&](int64_t g0, int64_t g1) {   // CSR rows in internal order
+        for (int64_t g = g0; g < g1; ++g) {
+            const int k = instOfRow[g];
+            const pdlpx_problem& p = probs[X.dev[k]];
+            const int R = X.rowOff[k], Cc = X.colOff[k];
+            const int i = X.yOld[g] - R;
+            int64_t w = rp[g];
+            for (int64_t e = p.csr_row_start[i]; e < p.csr_row_start[i + 1]; ++e, ++w) {
+                ci[w] = X.xNew[Cc + p.csr_col[e]];
+                rv[w] = p.csr_val[e];
+            }
+            rlo[g] = p.row_lower[i];
+            rhi[g] = p.row_upper[i];
+        }
+    });
+    parallelRanges(X.n, total, [&](int64_t g0, int64_t g1) {   // CSC columns in internal order
+        for (int64_t g = g0; g < g1; ++g) {
+            const int k = instOfCol[g];
+            const pdlpx_problem& p = probs[X.dev[k]];
+            const int R = X.rowOff[k], Cc = X.colOff[k];
+            const int j = X.xOld[g] - Cc;
+            int64_t w = cp[g];
+            for (int64_t e = p.csc_col_start[j]; e < p.csc_col_start[j + 1]; ++e, ++w) {
+                ri[w] = X.yNew[R + p.csc_row[e]];
+                cv[w] = p.csc_val[e];
+            }
+            c[g] = p.objective[j];
+            lo[g] = p.var_lower[j];
+            hi[g] = p.var_upper[j];
+        }
+    });
     hipStream_t s = X.stream;
     uploadOrient(X.A, rp, X.rowOff, ci, rv, s);
     uploadOrient(X.AT, cp, X.colOff, ri, cv, s);
