@@ -164,3 +164,42 @@ def test_batch_with_structural_corner_cases(precision):
         assert np.array_equal(s.primal, r.primal) and np.array_equal(s.row_dual, r.row_dual)
     for (lp, obj), r in zip(st.values(), res[1:-1]):
         assert r.status == "optimal" and abs(r.primal_objective - obj) <= 1e-6 * (1 + abs(obj))
+
+
+def test_update_vectors_equals_fresh_load():
+    """pdlpx_update_vectors: new costs and row bounds on a loaded matrix solve
+    exactly like a fresh load of the modified LP (bitwise), in a batch member
+    too; an invalid update is rejected and leaves the instance unchanged."""
+    from optimisation_solver_amd.solver import EngineError
+    base = box_lp(600, 1500, 5, seed=61)
+    g = np.random.default_rng(5)
+    c2 = base.c * (1 + 0.05 * g.standard_normal(base.n))
+    rlo2 = np.where(np.isfinite(base.rlo), base.rlo * 1.02, base.rlo)
+    rhi2 = np.where(np.isfinite(base.rhi), base.rhi * 1.02, base.rhi)
+    rhi2 = np.maximum(rhi2, rlo2)
+    moved = base.with_costs_and_rows(c2, rlo2, rhi2)
+    other = box_lp(300, 750, 5, seed=62)
+    o = _opts(1e-6)
+    with Engine() as e:
+        e.load(moved)
+        fresh = e.solve(o)
+    with Engine() as e:
+        e.load_batch([other, base])
+        before = e.solve_batch(o)
+        e.update_vectors(1, c2, rlo2, rhi2)
+        after = e.solve_batch(o)
+        bad = c2.copy()
+        bad[3] = np.nan
+        with pytest.raises(EngineError):
+            e.update_vectors(1, bad, None, None)
+        with pytest.raises(EngineError):
+            e.update_vectors(1, None, rhi2 + 1.0, rlo2)    # crossed row bounds
+        with pytest.raises(EngineError):
+            e.update_vectors(2, c2, None, None)             # no such instance
+        again = e.solve_batch(o)
+    print(f"fresh {fresh.iterations} its, updated {after[1].iterations} its", flush=True)
+    assert (after[1].status, after[1].iterations) == (fresh.status, fresh.iterations)
+    assert np.array_equal(after[1].primal, fresh.primal) and np.array_equal(after[1].row_dual, fresh.row_dual)
+    for a, b in ((before[0], after[0]), (after[0], again[0]), (after[1], again[1])):
+        assert a.iterations == b.iterations and np.array_equal(a.primal, b.primal)
+    assert not np.array_equal(before[1].primal, after[1].primal)
