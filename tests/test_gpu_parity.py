@@ -297,6 +297,29 @@ def test_invalid_problems_through_the_solver_facade(name):
     assert res[1].status == "optimal"
 
 
+def test_progress_callback():
+    """pdlpx_set_progress: called at termination checks with the running
+    count and a non-decreasing iteration; the solve is bitwise the same with
+    and without it, and unset it stays silent."""
+    lp = box_lp(2000, 5000, 5)
+    o = _opts(1e-6)
+    calls = []
+    with Engine() as e:
+        e.load(lp)
+        plain = e.solve(o)
+        e.set_progress(lambda run, it, kkt: calls.append((run, it, kkt)), 0.0)
+        seen = e.solve(o)
+        n = len(calls)
+        e.set_progress(None)
+        e.solve(o)
+    print(f"{n} progress calls, last {calls[-1] if calls else None}", flush=True)
+    assert n > 0 and len(calls) == n
+    assert all(run == 1 for run, _, _ in calls)
+    its = [it for _, it, _ in calls]
+    assert its == sorted(its) and its[-1] <= seen.iterations
+    assert seen.iterations == plain.iterations and np.array_equal(seen.primal, plain.primal)
+
+
 def test_graph_replay_equals_eager_launch():
     """The captured hipGraph replays the same kernels in the same order as the
     eager launch path: every replay (many per solve, with host work between
