@@ -77,17 +77,26 @@ struct DBuf {
 
 // Runs fn(begin, end) over [0, n) split into contiguous ranges on up to 16
 // host threads (one when the work is small).
-template <typename Fn>
-void parallelRanges(int64_t n, int64_t work, Fn fn) {
+int hostThreads(int64_t n, int64_t work) {
     const int hw = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
-    const int T = static_cast<int>(std::min<int64_t>({16, hw, std::max<int64_t>(1, work >> 20), std::max<int64_t>(1, n)}));
+    return static_cast<int>(std::min<int64_t>({16, hw, std::max<int64_t>(1, work >> 20), std::max<int64_t>(1, n)}));
+}
+
+// fn(t, begin, end) for t in [0, T): range t of [0, n) split T ways.
+template <typename Fn>
+void parallelChunks(int T, int64_t n, Fn fn) {
     if (T <= 1) {
-        fn(int64_t(0), n);
+        fn(0, int64_t(0), n);
         return;
     }
     std::vector<std::thread> pool;
-    for (int t = 0; t < T; ++t) pool.emplace_back(fn, n * t / T, n * (t + 1) / T);
+    for (int t = 0; t < T; ++t) pool.emplace_back(fn, t, n * t / T, n * (t + 1) / T);
     for (auto& th : pool) th.join();
+}
+
+template <typename Fn>
+void parallelRanges(int64_t n, int64_t work, Fn fn) {
+    parallelChunks(hostThreads(n, work), n, [&](int, int64_t a, int64_t b) { fn(a, b); });
 }
 
 int gridFor(int64_t n, int threads = kThreads) {
@@ -1563,20 +1572,38 @@ void buildSlices(pdlpx_ctx& X, const std::vector<int32_t>& cp, const std::vector
     }
     const int64_t Z = X.nnz;
     std::vector<int32_t> sid(Z), colOf(Z), perm(Z), sptr, rowCol, instStart(X.B + 1, 0);
-    for (int j = 0; j < X.n; ++j)
-        for (int32_t e = cp[j]; e < cp[j + 1]; ++e) colOf[e] = j;
+    parallelRanges(X.n, Z, [&](int64_t j0, int64_t j1) {
+        for (int64_t j = j0; j < j1; ++j)
+            for (int32_t e = cp[j]; e < cp[j + 1]; ++e) colOf[e] = static_cast<int32_t>(j);
+    });
     std::vector<int64_t> cnt;
     for (int k = 0; k < X.B; ++k) {
         const int R = X.rowOff[k], mk = X.rowOff[k + 1] - R;
         const int S = std::max(1, (mk + X.sliceRows - 1) / X.sliceRows);
         const int32_t e0 = cp[X.colOff[k]], e1 = cp[X.colOff[k + 1]];
-        cnt.assign(size_t(S) + 1, 0);
-        for (int32_t e = e0; e < e1; ++e) {
-            sid[e] = static_cast<int32_t>(int64_t(ri[e] - R) * S / mk);
-            ++cnt[sid[e] + 1];
-        }
-        for (int q = 0; q < S; ++q) cnt[q + 1] += cnt[q];
-        for (int32_t e = e0; e < e1; ++e) perm[e0 + cnt[sid[e]]++] = e;
+        // Stable counting sort of the instance's entries by slice: per-chunk
+        // counts, offsets in (slice, chunk) order, then each chunk scatters
+        // its entries in index order -- the serial sort's order exactly.
+        const int T = hostThreads(e1 - e0, e1 - e0);
+        cnt.assign(size_t(S) * T, 0);
+        parallelChunks(T, e1 - e0, [&](int t, int64_t a, int64_t b) {
+            int64_t* c = cnt.data() + size_t(t) * S;
+            for (int64_t e = e0 + a; e < e0 + b; ++e) {
+                sid[e] = static_cast<int32_t>(int64_t(ri[e] - R) * S / mk);
+                ++c[sid[e]];
+            }
+        });
+        int64_t run = 0;
+        for (int q = 0; q < S; ++q)
+            for (int t = 0; t < T; ++t) {
+                const int64_t c = cnt[size_t(t) * S + q];
+                cnt[size_t(t) * S + q] = run;
+                run += c;
+            }
+        parallelChunks(T, e1 - e0, [&](int t, int64_t a, int64_t b) {
+            int64_t* c = cnt.data() + size_t(t) * S;
+            for (int64_t e = e0 + a; e < e0 + b; ++e) perm[e0 + c[sid[e]]++] = static_cast<int32_t>(e);
+        });
         instStart[k] = static_cast<int32_t>(rowCol.size());
         for (int32_t q = e0; q < e1; ++q) {
             const int32_t e = perm[q];
@@ -1613,7 +1640,9 @@ void buildSlices(pdlpx_ctx& X, const std::vector<int32_t>& cp, const std::vector
     const int32_t R2 = static_cast<int32_t>(rowCol.size());
     sptr.push_back(static_cast<int32_t>(Z));
     std::vector<int32_t> sidx(Z), cptr(size_t(X.n) + 1, 0), cidx(R2);
-    for (int64_t q = 0; q < Z; ++q) sidx[q] = ri[perm[q]];
+    parallelRanges(Z, Z, [&](int64_t q0, int64_t q1) {
+        for (int64_t q = q0; q < q1; ++q) sidx[q] = ri[perm[q]];
+    });
     for (int32_t r = 0; r < R2; ++r) ++cptr[rowCol[r] + 1];
     for (int j = 0; j < X.n; ++j) cptr[j + 1] += cptr[j];
     std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
