@@ -458,6 +458,10 @@ def main() -> int:
                              "ms_per_launch": round(ms_d if dom[0] == "pdhg_primal_kernel" else ms_p, 5),
                              "achieved": round(gbs_d if dom[0] == "pdhg_primal_kernel" else gbs_p, 1)}}
     roof["gather_floor"] = gather_floor(eng, prec, args.kernel_reps, ms_p, ms_d)
+    roof["timing"] = ("hipEvents around each half in solve order (primal, dual, decision), averaged over "
+                      f"{args.kernel_reps} trials on the engine's stream; an interval includes its launch's "
+                      "dispatch (about 4 us), so it reads longer than the kernel-only durations of a rocprofv3 "
+                      "trace (profiles/pdhg_trial_kernels_*.txt)")
 
     # parity: re-score the GPU point with the oracle's KKT evaluator
     sys.path.insert(0, str(ROOT / "tests"))
