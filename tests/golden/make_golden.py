@@ -46,6 +46,9 @@ def cases():
         out.append((name, lp, opts(tol, **strict)))
     for name, (lp, _obj) in problems.structural().items():
         out.append((name, lp, opts(1e-8, **strict)))
+    for name, kw in problems.option_variants().items():
+        out.append((f"option_{name}", problems.random_feasible(300, 420, 21),
+                    opts(1e-6, iteration_limit=100000, use_feasibility_polishing=0, **kw)))
     lim = dict(strict, iteration_limit=20000)
     out.append(("infeasible_rows", problems.infeasible_rows(), opts(2e-6, **lim)))
     out.append(("infeasible_equalities", problems.infeasible_equalities(), opts(2e-6, **lim)))
