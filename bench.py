@@ -155,8 +155,13 @@ def bench_batch(args) -> int:
     bp, bd = trial_bytes(_Sz, *SIZES[pname])
     gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
     dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
+    workload = f"batch_{len(lps)}_box_lps_m500-20000"
+    traffic = None
+    tfile = ROOT / "profiles" / f"traffic_{workload}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get(dom[0])
     roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom[0],
+            "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
             "bytes_per_launch": int(dom[2]), "ms_per_launch": round(dom[1], 5)}
     roof["gather_floor"] = gather_floor(eng, prec, args.kernel_reps, ms_p, ms_d)
     eng.close()
@@ -201,7 +206,7 @@ def bench_batch(args) -> int:
         "metric": METRIC, "value": round(sec, 4), "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2), "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": DTYPES[pname], "data": "synthetic",
-        "config": {"workload": f"batch_{len(lps)}_box_lps_m500-20000", "precision": pname,
+        "config": {"workload": workload, "precision": pname,
                    "instances": len(lps), "sum_m": m_tot,
                    "sum_n": n_tot, "sum_nnz": Z, "tolerance": tol,
                    "optimal": int(sum(r.status == "optimal" for r in res)),
