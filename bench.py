@@ -10,7 +10,8 @@ CPU-runnable 2,000 x 5,000 LP.
 
 Printed (stdout, one JSON line): the contract fields plus
   roofline     the dominant trial kernel's algorithmic bytes per launch over
-               its average launch time (hipEvents on the engine's stream),
+               its average kernel time (hipEvents at the kernel's own start
+               and end on the engine's stream),
                against the 8 TB/s HBM peak;
   cpu_baseline the reference PDLP compiled from its own sources (oracle/_ref)
                timed on this host's cores on a bounded sample: set-up plus N
@@ -65,6 +66,35 @@ def gather_floor(eng, prec: int, reps: int, ms_p: float, ms_d: float) -> dict:
             "note": "bare gather pass (index, value, gathered element) over each half's nonzeros with the "
                     "trial kernels' row blocks; frac = floor / fused half (the sliced primal half also "
                     "runs its combine pass)"}
+
+
+def trial_roofline(eng, prec: int, reps: int, bp: float, bd: float, workload: str) -> dict:
+    """The roofline object for the dominant trial half: algorithmic bytes per
+    launch (`bp` primal, `bd` dual) over the half's mean kernel time, HBM
+    traffic from the counter summary in profiles/ when one exists, and the
+    bare-gather floor beside it."""
+    ms_p, ms_d = eng.time_trial(prec, reps)
+    gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
+    names = ("pdhg_primal_kernel", "pdhg_dual_kernel")
+    halves = {names[0]: (ms_p, bp, gbs_p), names[1]: (ms_d, bd, gbs_d)}
+    dom = names[0] if ms_p >= ms_d else names[1]
+    other = names[1] if dom == names[0] else names[0]
+    traffic = {}
+    tfile = ROOT / "profiles" / f"traffic_{workload}.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text())
+    ms, b, gbs = halves[dom]
+    roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get(dom), "kernel": dom,
+            "bytes_per_launch": int(b), "ms_per_launch": round(ms, 5),
+            "other_kernel": {"name": other, "ms_per_launch": round(halves[other][0], 5),
+                             "achieved": round(halves[other][2], 1), "traffic": traffic.get(other)}}
+    roof["gather_floor"] = gather_floor(eng, prec, reps, ms_p, ms_d)
+    roof["timing"] = ("hipEvents stamped with each half's own kernel start and end (hipExtLaunchKernel), in solve "
+                      f"order (primal, dual, decision), averaged over {reps} trials on the engine's stream: "
+                      "kernel-only durations, as a rocprofv3 kernel trace reports them "
+                      "(profiles/pdhg_trial_kernels_*.txt)")
+    return roof
 
 
 def cpu_baseline(lp, tol: float, gpu_iterations: int, sample_iters: int, threads: int,
@@ -145,25 +175,14 @@ def bench_batch(args) -> int:
         log(f"[bench] step {k}: {times[-1]:.3f}s statuses={st} iterations p50={np.median(its):.0f} max={its.max()}")
     sec = sum(times) / len(times)
     per = np.array([r.solve_time_seconds for r in res])
-    ms_p, ms_d = eng.time_trial(prec, args.kernel_reps)
     eng_tiling = eng.tiling()
     m_tot, n_tot = sum(lp.m for lp in lps), sum(lp.n for lp in lps)
 
     class _Sz:
         nnz, m, n = Z, m_tot, n_tot
 
-    bp, bd = trial_bytes(_Sz, *SIZES[pname])
-    gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
-    dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
     workload = f"batch_{len(lps)}_box_lps_m500-20000"
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{workload}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get(dom[0])
-    roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
-            "bytes_per_launch": int(dom[2]), "ms_per_launch": round(dom[1], 5)}
-    roof["gather_floor"] = gather_floor(eng, prec, args.kernel_reps, ms_p, ms_d)
+    roof = trial_roofline(eng, prec, args.kernel_reps, *trial_bytes(_Sz, *SIZES[pname]), workload)
     eng.close()
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as ol
@@ -456,27 +475,9 @@ def main() -> int:
             f"{res.dual_residual:.2e},{res.relative_gap:.2e}) fp32_its={res.diagnostics['fp32_iterations']}")
     sec = sum(times) / len(times)
 
-    # dominant kernel: per-launch time with hipEvents on the engine's stream,
-    # in the precision the bulk of the iterations ran in
-    ms_p, ms_d = eng.time_trial(prec, args.kernel_reps)
-    bp, bd = trial_bytes(lp, *SIZES[pname])
-    gbs_p, gbs_d = bp / (ms_p * 1e-3) / 1e9, bd / (ms_d * 1e-3) / 1e9
-    dom = ("pdhg_primal_kernel", ms_p, bp, gbs_p) if ms_p >= ms_d else ("pdhg_dual_kernel", ms_d, bd, gbs_d)
-    traffic = None
-    tfile = ROOT / "profiles" / f"traffic_{cfg['name']}.json"
-    if tfile.exists():
-        traffic = json.loads(tfile.read_text()).get(dom[0])
-    roof = {"bound": "hbm", "achieved": round(dom[3], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(dom[3] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom[0],
-            "bytes_per_launch": int(dom[2]), "ms_per_launch": round(dom[1], 5),
-            "other_kernel": {"name": "pdhg_dual_kernel" if dom[0] == "pdhg_primal_kernel" else "pdhg_primal_kernel",
-                             "ms_per_launch": round(ms_d if dom[0] == "pdhg_primal_kernel" else ms_p, 5),
-                             "achieved": round(gbs_d if dom[0] == "pdhg_primal_kernel" else gbs_p, 1)}}
-    roof["gather_floor"] = gather_floor(eng, prec, args.kernel_reps, ms_p, ms_d)
-    roof["timing"] = ("hipEvents around each half in solve order (primal, dual, decision), averaged over "
-                      f"{args.kernel_reps} trials on the engine's stream; an interval includes its launch's "
-                      "dispatch (about 4 us), so it reads longer than the kernel-only durations of a rocprofv3 "
-                      "trace (profiles/pdhg_trial_kernels_*.txt)")
+    # dominant kernel: per-launch kernel time in the precision the bulk of
+    # the iterations ran in
+    roof = trial_roofline(eng, prec, args.kernel_reps, *trial_bytes(lp, *SIZES[pname]), cfg["name"])
 
     # parity: re-score the GPU point with the oracle's KKT evaluator
     sys.path.insert(0, str(ROOT / "tests"))

@@ -223,10 +223,12 @@ int pdlpx_scaling(pdlpx_ctx* ctx, int32_t ruiz_iterations, int32_t use_precondit
 /*
  * Times the two halves of a PDHG trial as the solve issues them: on the state
  * the last solve left on the device, `reps` trials of (primal half, dual half,
- * linesearch decision) run in sequence on the context's stream, with hipEvents
- * between the kernels; writes the average milliseconds of the primal half
- * (A^T y + prox, both kernels when row-sliced) and of the dual half (A x' +
- * prox).  Each half therefore starts with the caches the previous one left, as
+ * linesearch decision) run in sequence on the context's stream; hipEvents
+ * take each half's own kernel start and end (hipExtLaunchKernel), so the
+ * times are kernel durations without the gaps between dispatches.  Writes the
+ * average milliseconds of the primal half (A^T y + prox; from the first
+ * kernel's start to the second's end when row-sliced) and of the dual half
+ * (A x' + prox).  Each half starts with the caches the previous one left, as
  * in the solve.  A profiling aid with no reference counterpart: the trials
  * advance the device iterate (the per-instance control state is restored
  * afterwards); solutions already returned are unaffected, and the next solve
@@ -241,8 +243,8 @@ int pdlpx_time_trial(pdlpx_ctx* ctx, int32_t precision, int32_t reps,
  * and thread mapping of the trial kernels but no row sums, prox, state or
  * partials.  ms_rows: over A (gathers x, as the dual half does); ms_cols:
  * over A^T, or its row-sliced copy when the load is sliced (gathers y, as the
- * primal half does).  Average milliseconds over `reps` passes, hipEvents on
- * the context's stream.  It is the floor the fused halves are measured
+ * primal half does).  Average milliseconds over `reps` passes, hipEvents
+ * taken at each pass's own kernel start and end on the context's stream.  It is the floor the fused halves are measured
  * against; the iterates are read, never written.  Needs a prior solve in
  * `precision`.  A profiling aid with no reference counterpart.
  */

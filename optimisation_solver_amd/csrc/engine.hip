@@ -17,6 +17,7 @@
 
 #include "pdlpx.h"
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +30,7 @@
 #include <random>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 using namespace pdlpx_dev;
@@ -410,17 +412,40 @@ T* satParts(pdlpx_ctx& X) {
     else return X.partS64.p;
 }
 
+// Launches `kernel` on `s`.  With a `start` or `stop` event, the event takes
+// the kernel's own start or end time (hipExtLaunchKernel), so an interval
+// between such events holds the kernels alone, not the gaps between
+// dispatches.  Without events it is a plain launch, the form graph capture
+// records.
+template <typename... P, typename... A>
+void launchOn(void (*kernel)(P...), int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop, A... a) {
+    if (!start && !stop) {
+        kernel<<<grid, kThreads, 0, s>>>(a...);
+        return;
+    }
+    std::tuple<P...> args{a...};
+    void* ptrs[sizeof...(P)];
+    std::apply([&](auto&... v) {
+        int i = 0;
+        ((ptrs[i++] = static_cast<void*>(&v)), ...);
+    }, args);
+    HIPCHK(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), dim3(grid), dim3(kThreads), ptrs, 0, s,
+                              start, stop, 0));
+}
+
 // The primal half of a trial for the instances in phase (V, T): one fused
 // row-block kernel over A^T, or the row-sliced pair when the context is sliced.
+// `start`/`stop` (optional) take the half's first kernel start and last kernel end.
 template <typename V, typename T>
-void launchPrimalHalf(pdlpx_ctx& X, hipStream_t s, const V* colVals, ColData<T> cols, Iterates<T> it) {
+void launchPrimalHalf(pdlpx_ctx& X, hipStream_t s, const V* colVals, ColData<T> cols, Iterates<T> it,
+                      hipEvent_t start = nullptr, hipEvent_t stop = nullptr) {
     if (X.sliced) {
-        k_primal_slices<V, T><<<X.SAT.nblk, kThreads, 0, s>>>(X.SAT.view<V>(satValues<V>(X)), it, X.st.p,
-                                                              satParts<T>(X));
-        k_primal_combine<V, T><<<X.nccs, kThreads, 0, s>>>(X.csegs.p, cols, it, X.st.p, X.cptr.p, X.cidx.p,
-                                                           satParts<T>(X), X.part_p.p);
+        launchOn(k_primal_slices<V, T>, X.SAT.nblk, s, start, nullptr, X.SAT.view<V>(satValues<V>(X)), it, X.st.p,
+                 satParts<T>(X));
+        launchOn(k_primal_combine<V, T>, X.nccs, s, nullptr, stop, X.csegs.p, cols, it, X.st.p, X.cptr.p, X.cidx.p,
+                 satParts<T>(X), X.part_p.p);
     } else {
-        k_primal<V, T><<<X.AT.tnblk(), kThreads, 0, s>>>(X.AT.tview<V>(colVals), cols, it, X.st.p, X.part_p.p);
+        launchOn(k_primal<V, T>, X.AT.tnblk(), s, start, stop, X.AT.tview<V>(colVals), cols, it, X.st.p, X.part_p.p);
     }
 }
 
@@ -2390,7 +2415,9 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
             h.target = std::numeric_limits<int64_t>::max();
         }
         HIPCHK(hipMemcpyAsync(X->st.p, probe.data(), sizeof(DevState) * B, hipMemcpyHostToDevice, s));
-        std::vector<hipEvent_t> ev(size_t(3) * reps, nullptr);   // before primal, before dual, after dual
+        // per trial: primal start, primal end, dual start, dual end, each taken
+        // from the kernels themselves
+        std::vector<hipEvent_t> ev(size_t(4) * reps, nullptr);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         auto seq = [&](auto vtag, auto ttag) {
             using V = decltype(vtag);
@@ -2413,9 +2440,9 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
                 wp = &X->w64;
                 ip = &X->it64;
             }
-            auto dual = [&] {
-                k_dual<V, T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<V>(vr), wp->rows(), ip->view(), X->st.p,
-                                                            X->part_d.p, X->part_i.p);
+            auto dual = [&](hipEvent_t e0, hipEvent_t e1) {
+                launchOn(k_dual<V, T>, X->A.nblk, s, e0, e1, X->A.view<V>(vr), wp->rows(), ip->view(), X->st.p,
+                         X->part_d.p, X->part_i.p);
             };
             auto decide = [&] {
                 k_decide<<<B, kReduceThreads, 0, s>>>(X->st.p, primalBegins(*X), X->A.ibeg.p, X->part_p.p,
@@ -2423,14 +2450,12 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
             };
             // one warm trial, then `reps` trials exactly as the solve issues them
             launchPrimalHalf<V, T>(*X, s, vc, wp->cols(), ip->view());
-            dual();
+            dual(nullptr, nullptr);
             decide();
             for (int r = 0; r < reps; ++r) {
-                HIPCHK(hipEventRecord(ev[3 * r], s));
-                launchPrimalHalf<V, T>(*X, s, vc, wp->cols(), ip->view());
-                HIPCHK(hipEventRecord(ev[3 * r + 1], s));
-                dual();
-                HIPCHK(hipEventRecord(ev[3 * r + 2], s));
+                hipEvent_t* e = &ev[4 * size_t(r)];
+                launchPrimalHalf<V, T>(*X, s, vc, wp->cols(), ip->view(), e[0], e[1]);
+                dual(e[2], e[3]);
                 decide();
             }
         };
@@ -2442,8 +2467,8 @@ int pdlpx_time_trial(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms_p
         double tp = 0.0, td = 0.0;
         for (int r = 0; r < reps; ++r) {
             float a = 0.f, b = 0.f;
-            HIPCHK(hipEventElapsedTime(&a, ev[3 * r], ev[3 * r + 1]));
-            HIPCHK(hipEventElapsedTime(&b, ev[3 * r + 1], ev[3 * r + 2]));
+            HIPCHK(hipEventElapsedTime(&a, ev[4 * r], ev[4 * r + 1]));
+            HIPCHK(hipEventElapsedTime(&b, ev[4 * r + 2], ev[4 * r + 3]));
             tp += a;
             td += b;
         }
@@ -2465,9 +2490,9 @@ int pdlpx_time_gathers(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms
         const OrientBuf& C = X->sliced ? X->SAT : X->AT;
         DBuf<double> out;
         out.alloc(size_t(std::max(X->A.nblk, C.nblk)) * kWaves);
-        // events between the passes, one host wait at the end: the passes run
-        // back to back as the trial kernels do inside a solve
-        std::vector<hipEvent_t> e(size_t(3) * reps, nullptr);
+        // events taken from each pass's own start and end, one host wait at the
+        // end: the passes run back to back as the trial kernels do inside a solve
+        std::vector<hipEvent_t> e(size_t(4) * reps, nullptr);
         for (auto& h : e) HIPCHK(hipEventCreate(&h));
         double tr = 0.0, tc = 0.0;
         auto run = [&](auto vtag, auto ttag) {
@@ -2488,22 +2513,23 @@ int pdlpx_time_gathers(pdlpx_ctx* X, int32_t precision, int32_t reps, double* ms
             if (!vr || !vcol || !ip->x[0].p || !ip->y[0].p)
                 throw Failure{PDLPX_E_STATE, "no solve in this precision has run"};
             T* o = reinterpret_cast<T*>(out.p);
-            auto rows = [&] { k_gather_probe<V, T><<<X->A.nblk, kThreads, 0, s>>>(X->A.view<V>(vr), ip->x[0].p, o); };
-            auto cols = [&] { k_gather_probe<V, T><<<C.nblk, kThreads, 0, s>>>(C.view<V>(vcol), ip->y[0].p, o); };
-            rows();
-            cols();
+            auto rows = [&](hipEvent_t e0, hipEvent_t e1) {
+                launchOn(k_gather_probe<V, T>, X->A.nblk, s, e0, e1, X->A.view<V>(vr), ip->x[0].p, o);
+            };
+            auto cols = [&](hipEvent_t e0, hipEvent_t e1) {
+                launchOn(k_gather_probe<V, T>, C.nblk, s, e0, e1, C.view<V>(vcol), ip->y[0].p, o);
+            };
+            rows(nullptr, nullptr);
+            cols(nullptr, nullptr);
             for (int r = 0; r < reps; ++r) {
-                HIPCHK(hipEventRecord(e[3 * r], s));
-                rows();
-                HIPCHK(hipEventRecord(e[3 * r + 1], s));
-                cols();
-                HIPCHK(hipEventRecord(e[3 * r + 2], s));
+                rows(e[4 * r], e[4 * r + 1]);
+                cols(e[4 * r + 2], e[4 * r + 3]);
             }
             HIPCHK(hipStreamSynchronize(s));
             for (int r = 0; r < reps; ++r) {
                 float a = 0.f, b = 0.f;
-                HIPCHK(hipEventElapsedTime(&a, e[3 * r], e[3 * r + 1]));
-                HIPCHK(hipEventElapsedTime(&b, e[3 * r + 1], e[3 * r + 2]));
+                HIPCHK(hipEventElapsedTime(&a, e[4 * r], e[4 * r + 1]));
+                HIPCHK(hipEventElapsedTime(&b, e[4 * r + 2], e[4 * r + 3]));
                 tr += a;
                 tc += b;
             }

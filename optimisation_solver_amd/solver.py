@@ -143,7 +143,8 @@ class Engine:
 
     def time_trial(self, precision: int = _native.FP64, reps: int = 20) -> tuple[float, float]:
         """Average ms per launch of the primal and dual trial kernels (hipEvents
-        on the engine's stream), on the state of the last solve."""
+        at the kernels' own start and end on the engine's stream), on the state
+        of the last solve."""
         a, b = C.c_double(), C.c_double()
         self._check(self.lib.pdlpx_time_trial(self.ctx, precision, reps, C.byref(a), C.byref(b)),
                     "pdlpx_time_trial")
