@@ -552,6 +552,15 @@ private:
         return out;
     }
 
+    // Host copy of red[q0 B, q1 B): one copy, one wait.
+    std::vector<double> pullRed(int q0, int q1) {
+        std::vector<double> h(size_t(q1 - q0) * B);
+        HIPCHK(hipMemcpyAsync(h.data(), X.red.p + size_t(q0) * B, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                              s));
+        HIPCHK(hipStreamSynchronize(s));
+        return h;
+    }
+
     template <typename T>
     const Working<T>& work() const;
     template <typename T>
@@ -716,8 +725,10 @@ private:
     // Scores the current (and, where fl[F_MODE] == 2, the averaged) iterate of
     // every instance with fl[F_MODE] != 0; leaves the original-coordinate
     // vectors in xo0/yo0 and xo1/yo1 and the phase-split modes in
-    // F_MODE64 / F_MODE32.
-    void score(std::vector<Metrics>& cur, std::vector<Metrics>& avg) {
+    // F_MODE64 / F_MODE32.  scoreLaunch queues the passes and their sums into
+    // red[0, 12B); scoreFinish turns a host copy of those sums (`h`, from
+    // red[0]) into the metrics.
+    void scoreLaunch() {
         std::fill(fl[F_MODE64].begin(), fl[F_MODE64].end(), 0);
         std::fill(fl[F_MODE32].begin(), fl[F_MODE32].end(), 0);
         for (int k = 0; k < B; ++k)
@@ -739,21 +750,32 @@ private:
         std::vector<RQ> q;
         for (int i = 0; i < 6; ++i) q.push_back({X.part_rows.p + size_t(i) * nr, X.A.ibeg.p});
         for (int i = 0; i < 6; ++i) q.push_back({X.part_cols.p + size_t(i) * nc, X.AT.ibeg.p});
-        const auto v = reduce(q);
+        reduce(q, false, 0);
+    }
+    void scoreFinish(const std::vector<double>& h, std::vector<Metrics>& cur, std::vector<Metrics>& avg) {
+        auto v = [&](int q, int k) { return h[size_t(q) * B + k]; };
         cur.assign(B, Metrics{});
         avg.assign(B, Metrics{});
         for (int k = 0; k < B; ++k) {
             if (!fl[F_MODE][k]) continue;
-            cur[k] = finish(k, v[0][k], v[1][k], v[2][k], v[6][k], v[7][k], v[8][k]);
-            avg[k] = finish(k, v[3][k], v[4][k], v[5][k], v[9][k], v[10][k], v[11][k]);
+            cur[k] = finish(k, v(0, k), v(1, k), v(2, k), v(6, k), v(7, k), v(8, k));
+            avg[k] = finish(k, v(3, k), v(4, k), v(5, k), v(9, k), v(10, k), v(11, k));
             ++I[k].checks;
         }
     }
+    void score(std::vector<Metrics>& cur, std::vector<Metrics>& avg) {
+        scoreLaunch();
+        scoreFinish(pullRed(0, 12), cur, avg);
+    }
 
     // Ray tests (infeasibility.cpp:51-192) for the instances with
-    // fl[F_MODE] != 0, on the phase split score() left in F_MODE64/F_MODE32.
-    // Returns per instance 0, 1 = Farkas ray (in ry), 2 = improving ray (in rx).
-    std::vector<int> rayTests() {
+    // fl[F_MODE] != 0, on the phase split scoreLaunch left in F_MODE64/F_MODE32.
+    // rayLaunch queues the passes: the direction norms into red[16B, 20B),
+    // each instance's pair picked on the device into red[20B, 22B), and the
+    // certificate sums into red[22B, 29B).  rayFinish reads a host copy of
+    // red[0, 29B) (`h`) and returns per instance 0, 1 = Farkas ray (in ry),
+    // 2 = improving ray (in rx).
+    void rayLaunch() {
         const int ns = nsegs();
         if (anyOf(fl[F_MODE64]))
             k_ray_dir<double><<<ns, kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flagDev(F_MODE64), X.cs.p, X.rs.p,
@@ -762,23 +784,11 @@ private:
             k_ray_dir<float><<<ns, kThreads, 0, s>>>(X.segs.p, X.ncs, X.st.p, flagDev(F_MODE32), X.cs.p, X.rs.p,
                                                      X.it32.view(), X.rx.p, X.ry.p, X.part_seg2.p);
         HIPCHK(hipGetLastError());
-        // squared norms (both precisions' partials) into red[16B..18B)
+        // squared norms (both precisions' partials) into red[16B..20B)
         reduce({{X.part_seg.p, X.colSegBeg.p}, {X.part_seg.p, X.rowSegBegAll.p}, {X.part_seg2.p, X.colSegBeg.p},
                 {X.part_seg2.p, X.rowSegBegAll.p}},
                false, 16);
-        // per instance pick the precision that wrote the partials
-        std::vector<double> pick(2 * size_t(B));
-        std::vector<double> flat(4 * size_t(B));
-        HIPCHK(hipMemcpyAsync(flat.data(), X.red.p + size_t(16) * B, sizeof(double) * flat.size(),
-                              hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        for (int k = 0; k < B; ++k) {
-            const bool f32 = fl[F_MODE32][k] != 0;
-            pick[k] = flat[size_t(f32 ? 2 : 0) * B + k];
-            pick[B + k] = flat[size_t(f32 ? 3 : 1) * B + k];
-        }
-        HIPCHK(hipMemcpyAsync(X.red.p + size_t(20) * B, pick.data(), sizeof(double) * pick.size(),
-                              hipMemcpyHostToDevice, s));
+        k_pick_norms<<<gridFor(B), kThreads, 0, s>>>(B, flagDev(F_MODE32), X.red.p);
         const double* nx2 = X.red.p + size_t(20) * B;
         const double* ny2 = X.red.p + size_t(21) * B;
         k_farkas_rows<<<X.nrs, kThreads, 0, s>>>(X.segs.p, X.ncs, flagDev(F_MODE), X.ry.p, ny2, X.rlo.p, X.rhi.p,
@@ -791,30 +801,34 @@ private:
                                                  X.rhi.p, X.part_rows.p);
         HIPCHK(hipGetLastError());
         const int nc = X.AT.nblk;
-        const auto v = reduce({{X.part_seg2.p, X.rowSegBeg.p},
-                               {X.part_seg2.p + X.nrs, X.rowSegBeg.p},
-                               {X.part_cols.p, X.AT.ibeg.p},
-                               {X.part_cols.p + nc, X.AT.ibeg.p},
-                               {X.part_seg3.p, X.colSegBeg.p},
-                               {X.part_seg3.p + X.ncs, X.colSegBeg.p},
-                               {X.part_rows.p, X.A.ibeg.p}});
+        reduce({{X.part_seg2.p, X.rowSegBeg.p},
+                {X.part_seg2.p + X.nrs, X.rowSegBeg.p},
+                {X.part_cols.p, X.AT.ibeg.p},
+                {X.part_cols.p + nc, X.AT.ibeg.p},
+                {X.part_seg3.p, X.colSegBeg.p},
+                {X.part_seg3.p + X.ncs, X.colSegBeg.p},
+                {X.part_rows.p, X.A.ibeg.p}},
+               false, 22);
+    }
+    std::vector<int> rayFinish(const std::vector<double>& h) const {
+        auto v = [&](int q, int k) { return h[size_t(22 + q) * B + k]; };
         std::vector<int> out(B, 0);
         for (int k = 0; k < B; ++k) {
             if (!fl[F_MODE][k]) continue;
             const InstInfo& in = X.info[X.dev[k]];
-            const double nx = std::sqrt(pick[k]), ny = std::sqrt(pick[B + k]);
+            const double nx = std::sqrt(h[size_t(20) * B + k]), ny = std::sqrt(h[size_t(21) * B + k]);
             if (in.m > 0 && ny > 1e-300 && std::isfinite(ny)) {
-                const double value = v[3][k] - v[1][k];
+                const double value = v(3, k) - v(1, k);
                 if (std::isfinite(value) && value > 0.0 &&
-                    std::sqrt(v[2][k] + v[0][k]) / value <= o.infeasibility_tolerance) {
+                    std::sqrt(v(2, k) + v(0, k)) / value <= o.infeasibility_tolerance) {
                     out[k] = 1;
                     continue;
                 }
             }
             if (in.n > 0 && nx > 1e-300 && std::isfinite(nx)) {
-                const double rate = v[5][k];
+                const double rate = v(5, k);
                 if (std::isfinite(rate) && rate < 0.0 &&
-                    std::sqrt(v[4][k] + v[6][k]) / (-rate) <= o.infeasibility_tolerance)
+                    std::sqrt(v(4, k) + v(6, k)) / (-rate) <= o.infeasibility_tolerance)
                     out[k] = 2;
             }
         }
@@ -1210,8 +1224,13 @@ void Run::checkMain(const std::vector<int>& atCheck) {
     }
     std::fill(fl[F_MODE].begin(), fl[F_MODE].end(), 0);
     for (int k : atCheck) fl[F_MODE][k] = (o.use_averaging && hs[k].avg_total != 0.0) ? 2 : 1;
+    // The scoring passes and the ray tests go out together, with one host
+    // wait for all of their sums.
     std::vector<Metrics> cur, avg;
-    score(cur, avg);
+    scoreLaunch();
+    if (o.detect_infeasibility) rayLaunch();
+    const std::vector<double> sums = pullRed(0, o.detect_infeasibility ? 29 : 12);
+    scoreFinish(sums, cur, avg);
     std::fill(fl[F_SEL].begin(), fl[F_SEL].end(), 0);
     for (int k : atCheck) {   // best candidate: current, then average
         if (fl[F_MODE][k] == 1) avg[k] = cur[k];
@@ -1226,7 +1245,7 @@ void Run::checkMain(const std::vector<int>& atCheck) {
     }
     selectInto(X.bestx.p, X.besty.p, X.xo0.p, X.yo0.p, X.xo1.p, X.yo1.p);
     if (o.detect_infeasibility) {
-        const std::vector<int> ray = rayTests();
+        const std::vector<int> ray = rayFinish(sums);
         bool anyRay = false;
         for (int k : atCheck) {
             if (!ray[k]) continue;
@@ -1721,7 +1740,7 @@ void allocWork(pdlpx_ctx& X, bool fp32) {
     X.part_seg.alloc(2 * ns);
     X.part_seg2.alloc(2 * ns);
     X.part_seg3.alloc(2 * ns);
-    X.red.alloc(24 * size_t(B));
+    X.red.alloc(32 * size_t(B));
     X.flags.alloc(F_COUNT * size_t(B));
     X.st.alloc(B);
     if (fp32) {

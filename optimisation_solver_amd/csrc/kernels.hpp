@@ -816,6 +816,20 @@ __global__ __launch_bounds__(kThreads) void k_reduce_seg(SegReduce job) {
     if (threadIdx.x == 0) job.dst[q][k] = v[0];
 }
 
+// Squared norms of each instance's ray directions, from the partials of the
+// precision its iterate is in: red[20B + k], red[21B + k] take the FP32 pair
+// red[18B + k], red[19B + k] where mode32[k] != 0, else the FP64 pair
+// red[16B + k], red[17B + k].
+__global__ __launch_bounds__(kThreads) void k_pick_norms(int B, const int32_t* __restrict__ mode32,
+                                                         double* __restrict__ red) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B) return;
+    const size_t b = static_cast<size_t>(B);
+    const bool f32 = mode32[k] != 0;
+    red[20 * b + k] = red[(f32 ? 18 : 16) * b + k];
+    red[21 * b + k] = red[(f32 ? 19 : 17) * b + k];
+}
+
 // ---------------------------------------------------------------------------
 // Ray certificates (infeasibility.cpp:51-192) for the instances with mode != 0.
 // ---------------------------------------------------------------------------
