@@ -430,8 +430,8 @@ def main() -> int:
                           "reps": args.kernel_reps}), flush=True)
         return 0
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding is a design note, not an implementation)", file=sys.stderr)
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the engine runs on one GPU; sharding over several is designed, not implemented)", file=sys.stderr)
         return 2
 
     from optimisation_solver_amd import CONFIGS, Engine, default_options
