@@ -1,0 +1,91 @@
+"""Host-side logic of bench.py, with no device: the one-GPU refusal, the
+roofline object built from the two halves' kernel times, and the
+algorithmic byte counts the committed bench lines were computed with."""
+from __future__ import annotations
+
+import json
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+import bench  # noqa: E402
+
+
+class _StubEngine:
+    """time_trial / time_gathers with fixed milliseconds."""
+
+    def __init__(self, ms_p, ms_d, g_rows, g_cols):
+        self.t = (ms_p, ms_d)
+        self.g = (g_rows, g_cols)
+        self.calls = []
+
+    def time_trial(self, prec, reps):
+        self.calls.append(("trial", prec, reps))
+        return self.t
+
+    def time_gathers(self, prec, reps):
+        self.calls.append(("gathers", prec, reps))
+        return self.g
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""   # no JSON line for an unsupported run
+
+
+@pytest.mark.parametrize("ms_p,ms_d,dom", [(0.040, 0.030, "pdhg_primal_kernel"), (0.020, 0.050, "pdhg_dual_kernel")])
+def test_trial_roofline_picks_the_slower_half(ms_p, ms_d, dom):
+    bp, bd = 1.2e8, 0.9e8
+    e = _StubEngine(ms_p, ms_d, 0.9 * ms_d, 0.8 * ms_p)
+    roof = bench.trial_roofline(e, 2, 7, bp, bd, "no_such_workload")
+    assert e.calls == [("trial", 2, 7), ("gathers", 2, 7)]
+    ms, b = (ms_p, bp) if dom == "pdhg_primal_kernel" else (ms_d, bd)
+    other = "pdhg_dual_kernel" if dom == "pdhg_primal_kernel" else "pdhg_primal_kernel"
+    assert roof["kernel"] == dom and roof["other_kernel"]["name"] == other
+    assert roof["bytes_per_launch"] == int(b)
+    assert roof["achieved"] == pytest.approx(b / (ms * 1e-3) / 1e9, abs=0.05)
+    assert roof["frac"] == pytest.approx(roof["achieved"] / 8000.0, abs=1e-4)
+    assert roof["peak"] == 8000.0 and roof["unit"] == "GB/s" and roof["bound"] == "hbm"
+    assert roof["traffic"] is None and roof["other_kernel"]["traffic"] is None   # no counter summary
+    assert roof["gather_floor"]["primal_frac"] == pytest.approx(0.8, abs=1e-3)
+    assert roof["gather_floor"]["dual_frac"] == pytest.approx(0.9, abs=1e-3)
+
+
+def test_trial_roofline_reads_the_committed_traffic():
+    name = "c1_zipf_200000x500000"
+    t = json.loads((ROOT / "profiles" / f"traffic_{name}.json").read_text())
+    e = _StubEngine(0.036, 0.033, 0.03, 0.03)
+    roof = bench.trial_roofline(e, 0, 20, 9.1e7, 8.0e7, name)
+    assert roof["traffic"] == t["pdhg_primal_kernel"]
+    assert roof["other_kernel"]["traffic"] == t["pdhg_dual_kernel"]
+
+
+@pytest.mark.parametrize("fname,m,n,prec", [
+    ("bench_c1_zipf_200k_fp64.json", 200_000, 500_000, "fp64"),
+    ("bench_c2_zipf_1M_mixed.json", 1_000_000, 2_000_000, "mixed"),
+])
+def test_committed_bench_lines_use_the_algorithmic_byte_model(fname, m, n, prec):
+    d = json.loads((ROOT / "profiles" / fname).read_text())
+    cfg = d["config"]
+    assert (cfg["m"], cfg["n"]) == (m, n)
+
+    class Sz:
+        pass
+
+    Sz.m, Sz.n, Sz.nnz = cfg["m"], cfg["n"], cfg["nnz"]
+    bp, bd = bench.trial_bytes(Sz, *bench.SIZES[prec])
+    r = d["roofline"]
+    want = bp if r["kernel"] == "pdhg_primal_kernel" else bd
+    assert r["bytes_per_launch"] == int(want)
+    # achieved = bytes / kernel time; frac = achieved / 8 TB/s
+    assert r["achieved"] == pytest.approx(want / (r["ms_per_launch"] * 1e-3) / 1e9, rel=1e-3)
+    assert r["frac"] == pytest.approx(r["achieved"] / 8000.0, abs=2e-4)
+    # the bound cannot exceed the bytes every nonzero must move at least once
+    assert want > cfg["nnz"] * (bench.SIZES[prec][0] + 4)
