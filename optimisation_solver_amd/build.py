@@ -5,6 +5,8 @@
                                            kernels (hipcc)
   optimisation_solver_amd/liblpgen.so      instance generators (g++)
   tests/cpp/test_dropin                    C++ drop-in tests (g++, links the engine)
+  tests/cpp/test_abi_paths                 C ABI paths beyond the drop-in (g++, links
+                                           the engine and the generators)
   oracle/liboracle.so, oracle/_ref/...     CPU checker (make -C oracle), and the
                                            reference's unit tests built against
                                            the drop-in (make -C oracle dropin)
@@ -36,6 +38,7 @@ HIP_FLAGS = [
 ENGINE_SO = PKG / "libpdlp_amd.so"
 LPGEN_SO = PKG / "liblpgen.so"
 DROPIN_TEST = ROOT / "tests" / "cpp" / "test_dropin"
+ABI_TEST = ROOT / "tests" / "cpp" / "test_abi_paths"
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> subprocess.Popen:
@@ -81,6 +84,13 @@ def build_dropin_test() -> subprocess.Popen:
                  "-pthread"])
 
 
+def build_abi_test() -> subprocess.Popen:
+    return _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
+                 "-o", ABI_TEST, ROOT / "tests" / "cpp" / "test_abi_paths.cpp",
+                 f"-L{PKG}", "-lpdlp_amd", "-llpgen", f"-Wl,-rpath,{PKG}",
+                 "-Wl,-rpath,$ORIGIN/../../optimisation_solver_amd", "-pthread"])
+
+
 def build_oracle() -> list[subprocess.Popen]:
     procs = [_run(["make", "-s", "-B", "-C", ROOT / "oracle"])]
     if Path("/root/reference/pdlp_engine/src").is_dir():
@@ -97,6 +107,8 @@ def build_all() -> None:
     later = []
     if (ROOT / "tests" / "cpp" / "test_dropin.cpp").exists():
         later.append(build_dropin_test())
+    if (ROOT / "tests" / "cpp" / "test_abi_paths.cpp").exists():
+        later.append(build_abi_test())
     if Path("/root/reference/pdlp_engine/tests/test_pdlp.cpp").is_file():
         # the reference's own unit tests against the drop-in headers and the engine
         later.append(_run(["make", "-s", "-B", "-C", ROOT / "oracle", "dropin"]))
