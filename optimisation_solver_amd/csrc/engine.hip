@@ -412,25 +412,23 @@ T* satParts(pdlpx_ctx& X) {
     else return X.partS64.p;
 }
 
-// Launches `kernel` on `s`.  With a `start` or `stop` event, the event takes
-// the kernel's own start or end time (hipExtLaunchKernel), so an interval
-// between such events holds the kernels alone, not the gaps between
-// dispatches.  Without events it is a plain launch, the form graph capture
-// records.
+// Launches `kernel` on `s` through the launch API, with the arguments
+// converted to the kernel's parameter types.  With a `start` or `stop` event,
+// the event takes the kernel's own start or end time (hipExtLaunchKernel), so
+// an interval between such events holds the kernels alone, not the gaps
+// between dispatches.  Without events it is hipLaunchKernel, which graph
+// capture records like any launch.
 template <typename... P, typename... A>
 void launchOn(void (*kernel)(P...), int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop, A... a) {
-    if (!start && !stop) {
-        kernel<<<grid, kThreads, 0, s>>>(a...);
-        return;
-    }
     std::tuple<P...> args{a...};
     void* ptrs[sizeof...(P)];
     std::apply([&](auto&... v) {
         int i = 0;
         ((ptrs[i++] = static_cast<void*>(&v)), ...);
     }, args);
-    HIPCHK(hipExtLaunchKernel(reinterpret_cast<const void*>(kernel), dim3(grid), dim3(kThreads), ptrs, 0, s,
-                              start, stop, 0));
+    const void* fn = reinterpret_cast<const void*>(kernel);
+    if (!start && !stop) HIPCHK(hipLaunchKernel(fn, dim3(grid), dim3(kThreads), ptrs, 0, s));
+    else HIPCHK(hipExtLaunchKernel(fn, dim3(grid), dim3(kThreads), ptrs, 0, s, start, stop, 0));
 }
 
 // The primal half of a trial for the instances in phase (V, T): one fused
