@@ -54,6 +54,8 @@ def cases():
     out.append(("infeasible_equalities", problems.infeasible_equalities(), opts(2e-6, **lim)))
     out.append(("unbounded", problems.unbounded(), opts(2e-6, **lim)))
     out.append(("unbounded_no_ray_rows", problems.unbounded_no_ray_rows(), opts(2e-6, **dict(strict, iteration_limit=2000))))
+    out.append(("infeasible_rows_no_detection", problems.infeasible_rows(),
+                opts(2e-6, **dict(strict, iteration_limit=2000, detect_infeasibility=0))))
     conv = dict(iteration_limit=60000, termination_check_frequency=40, use_feasibility_polishing=0)
     out.append(("random_feasible_21", problems.random_feasible(300, 420, 21), opts(1e-8, **conv)))
     out.append(("random_feasible_33_fixed", problems.random_feasible(260, 360, 33),

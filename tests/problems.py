@@ -86,6 +86,7 @@ def option_variants():
         "check_every_7": dict(termination_check_frequency=7),
         "step_safety_0.5": dict(initial_step_safety=0.5),
         "one_step_trial": dict(maximum_step_trials=1),
+        "no_infeasibility_detection": dict(detect_infeasibility=0),
     }
 
 

@@ -238,6 +238,23 @@ def test_infeasible_and_unbounded_statuses():
         assert e.solve(o2).status != "optimal"
 
 
+def test_infeasible_with_detection_off():
+    """detect_infeasibility = 0 (pdlp_options.h): the checks skip the ray
+    tests, so an infeasible LP runs to the iteration limit with no ray, as the
+    oracle does (pdlp_solver.cpp:378-390 runs the tests only when enabled)."""
+    o = _opts(2e-6, iteration_limit=2000, termination_check_frequency=50, polishing_iterations=2000,
+              detect_infeasibility=0)
+    lp = problems.infeasible_rows()
+    with Engine() as e:
+        e.load(lp)
+        r = e.solve(o)
+    cpu = ol.solve(lp, o)
+    assert cpu.status_name == "iteration_limit"
+    assert r.status == cpu.status_name and r.status_message == cpu.d["message"]
+    assert r.dual_ray is None and r.primal_ray is None
+    assert r.iterations >= 2000
+
+
 @pytest.mark.parametrize("precision", [1, 2])
 def test_infeasible_and_unbounded_reduced_precision(precision):
     """Infeasibility and unboundedness certificates with FP32 values: the ray

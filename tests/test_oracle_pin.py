@@ -203,8 +203,9 @@ def test_oracle_structural_optima(name):
 @pytest.mark.parametrize("name", list(problems.option_variants()))
 def test_oracle_bitwise_vs_reference_option_variants(name):
     """Every PdlpOptions switch the reference exposes (scaling, averaging,
-    restarts, linesearch, primal weight, check frequency, trial budget): the
-    oracle's trajectory equals the compiled reference's bit for bit."""
+    restarts, linesearch, primal weight, check frequency, trial budget,
+    infeasibility detection): the oracle's trajectory equals the compiled
+    reference's bit for bit."""
     lp = problems.random_feasible(300, 420, 21)
     o = opts(1e-6, iteration_limit=100000, use_feasibility_polishing=0, **problems.option_variants()[name])
     a = ol.solve(lp, o, "oracle")
