@@ -87,6 +87,11 @@ def option_variants():
         "step_safety_0.5": dict(initial_step_safety=0.5),
         "one_step_trial": dict(maximum_step_trials=1),
         "no_infeasibility_detection": dict(detect_infeasibility=0),
+        "restart_decays": dict(restart_sufficient_decay=0.5, restart_necessary_decay=0.95,
+                               restart_artificial_fraction=0.2),
+        "weight_clamp": dict(minimum_primal_weight=0.5, maximum_primal_weight=2.0),
+        "min_step_scale_0.9": dict(minimum_step_scale=0.9),
+        "power_iterations_15": dict(use_diagonal_preconditioning=0, power_iteration_count=15),
     }
 
 

@@ -203,7 +203,8 @@ def test_oracle_structural_optima(name):
 @pytest.mark.parametrize("name", list(problems.option_variants()))
 def test_oracle_bitwise_vs_reference_option_variants(name):
     """Every PdlpOptions switch the reference exposes (scaling, averaging,
-    restarts, linesearch, primal weight, check frequency, trial budget,
+    restarts and their decay fractions, linesearch, primal weight and its
+    clamps, minimum step, power iterations, check frequency, trial budget,
     infeasibility detection): the oracle's trajectory equals the compiled
     reference's bit for bit."""
     lp = problems.random_feasible(300, 420, 21)
