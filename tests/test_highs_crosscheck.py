@@ -1,0 +1,70 @@
+"""The oracle and the engine against an independent LP solver.
+
+tests/golden/highs_objectives.json holds HiGHS's optima (scipy's linprog,
+method="highs"; written by tests/golden/make_highs.py) for seeded instances
+of this repository's generators.  A PDLP point at relative KKT tolerance
+`tol` has a relative gap below `tol` and nearly feasible rows, so its
+objective lies within a few `tol` of the optimum; HiGHS's own default
+feasibility tolerance (1e-7) bounds the fixture's accuracy.  The bar is
+|p - p*| <= (5 tol + 2e-7) (1 + |p*|).
+"""
+from __future__ import annotations
+
+import json
+import sys
+from pathlib import Path
+
+import pytest
+
+sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+
+import oracle_lib as ol  # noqa: E402
+from make_highs import instances  # noqa: E402
+from optimisation_solver_amd import default_options  # noqa: E402
+
+HIGHS = json.loads((Path(__file__).resolve().parent / "golden" / "highs_objectives.json").read_text())["instances"]
+LPS = instances()
+CPU_CASES = [("c0_box_2000x5000", 1e-6), ("c0_box_2000x5000", 1e-8), ("box_1500x3750_s9", 1e-6),
+             ("random_feasible_300x420_s21", 1e-6), ("random_feasible_300x420_s21", 1e-8),
+             ("random_feasible_260x360_s33", 1e-6), ("random_feasible_260x360_s33", 1e-8)]
+GPU_CASES = [(n, 1e-6) for n in HIGHS] + [(n, 1e-8) for n in HIGHS if n.startswith(("c0", "box", "random"))]
+
+
+def _opts(tol, precision=0):
+    return default_options(primal_tolerance=tol, dual_tolerance=tol, gap_tolerance=tol, thread_count=1,
+                           iteration_limit=300000, precision=precision)
+
+
+def _close(p, name, tol):
+    ps = HIGHS[name]["objective"]
+    assert abs(p - ps) <= (5 * tol + 2e-7) * (1 + abs(ps)), (name, tol, p, ps)
+
+
+def test_fixture_instances_are_the_generated_ones():
+    for name, lp in LPS.items():
+        h = HIGHS[name]
+        assert h["status"] == 0 and h["objective"] is not None
+        assert (h["m"], h["n"], h["nnz"], h["sha256"]) == (lp.m, lp.n, int(lp.rp[-1]), lp.sha256()), name
+
+
+@pytest.mark.parametrize("name,tol", CPU_CASES)
+def test_oracle_reaches_the_highs_optimum(name, tol):
+    r = ol.solve(LPS[name], _opts(tol))
+    assert r.status_name == "optimal"
+    _close(r.primal_objective, name, tol)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,tol", GPU_CASES)
+def test_engine_reaches_the_highs_optimum(name, tol):
+    from optimisation_solver_amd import Engine
+
+    for precision in ((0, 2) if tol >= 1e-6 else (0,)):
+        print(f"highs cross-check {name} tol {tol:g} precision {precision}", flush=True)
+        with Engine() as e:
+            e.load(LPS[name], precision)
+            r = e.solve(_opts(tol, precision))
+        assert r.status == "optimal", (name, precision, r.status_message)
+        _close(r.primal_objective, name, tol)
+        m = ol.evaluate(LPS[name], _opts(tol), r.primal, r.row_dual)   # the oracle's KKT on the GPU point
+        assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= tol
