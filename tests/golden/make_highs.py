@@ -15,9 +15,6 @@ import sys
 from pathlib import Path
 
 import numpy as np
-import scipy
-import scipy.sparse as sp
-from scipy.optimize import linprog
 
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
@@ -45,6 +42,9 @@ def highs(lp) -> dict:
     """min c x + offset s.t. rlo <= A x <= rhi, lo <= x <= hi, as linprog's
     A_ub x <= b_ub (finite upper and negated finite lower row bounds of
     inequality rows) and A_eq x = b_eq (rows with rlo == rhi)."""
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+
     a = sp.csr_matrix((lp.rv, lp.ci, lp.rp), shape=(lp.m, lp.n))
     eq = np.isfinite(lp.rlo) & (lp.rlo == lp.rhi)
     up = ~eq & np.isfinite(lp.rhi)
@@ -61,6 +61,8 @@ def highs(lp) -> dict:
 
 
 def main() -> None:
+    import scipy
+
     out = {"solver": f"HiGHS via scipy {scipy.__version__} linprog(method='highs'), default tolerances",
            "instances": {}}
     for name, lp in instances().items():
