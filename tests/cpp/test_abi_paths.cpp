@@ -2,11 +2,13 @@
 // driven from C++: a batch with an empty and an invalid member in mixed
 // precision (with compaction as instances finish), the autotuner,
 // pdlpx_update_vectors with warm starts, row slicing of a batch, the
-// profiling hooks, SpMV and scaling of a power-law LP, and a Farkas ray.
+// profiling hooks, SpMV and scaling of a power-law LP, a Farkas ray, and the
+// warm-start network's ABI (include/pdlpx_mlp.h).
 // Checks return codes and statuses; numerical parity is the Python tests'
 // job.  Built by optimisation_solver_amd/build.py (and, with the host code
 // under AddressSanitizer, by tests/cpp/asan.sh); run from the repository root.
 #include "pdlpx.h"
+#include "pdlpx_mlp.h"
 
 #include <cmath>
 #include <cstdio>
@@ -191,6 +193,51 @@ int main() {
                one.status == PDLPX_INFEASIBLE && one.has_dual_ray,
            "infeasible with ray");
     pdlpx_destroy(X);
+
+    // ---- the warm-start network: 64 -> 128 -> 64
+    pdlpx_mlp* N = nullptr;
+    const int32_t bad_dims[3] = {64, 100, 64};
+    EXPECT(pdlpx_mlp_create(&N, 0, 2, bad_dims, 128, 1) != 0 && N == nullptr, "network: 100-wide layer rejected");
+    const int32_t dims[3] = {64, 128, 64};
+    EXPECT(pdlpx_mlp_create(&N, 0, 2, dims, 128, 7) == 0 && N != nullptr, "network: create");
+    std::vector<float> W0(128 * 64), b0(128), W0b(128 * 64), b0b(128);
+    EXPECT(pdlpx_mlp_get_params(N, 0, W0.data(), b0.data()) == 0, "network: get_params");
+    EXPECT(pdlpx_mlp_set_params(N, 0, W0.data(), b0.data()) == 0 &&
+               pdlpx_mlp_get_params(N, 0, W0b.data(), b0b.data()) == 0 && W0 == W0b && b0 == b0b,
+           "network: parameters round-trip");
+    std::vector<float> x(256 * 64), tgt(256 * 64), y(5 * 64);
+    for (size_t q = 0; q < x.size(); ++q) {
+        x[q] = static_cast<float>(std::sin(0.37 * static_cast<double>(q)));
+        tgt[q] = 0.5f * x[q];   // a learnable target: half the input
+    }
+    EXPECT(pdlpx_mlp_forward(N, 5, x.data(), y.data()) == 0 && std::isfinite(y[0]) && std::isfinite(y.back()),
+           "network: forward of 5 rows (padded)");
+    double loss = 0.0;
+    std::vector<float> dW0(128 * 64), db0(128), dW1(64 * 128), db1(64);
+    float* dW[2] = {dW0.data(), dW1.data()};
+    float* db[2] = {db0.data(), db1.data()};
+    EXPECT(pdlpx_mlp_gradients(N, 64, x.data(), tgt.data(), &loss, dW, db) == 0 && std::isfinite(loss) && loss > 0,
+           "network: gradients of one batch");
+    EXPECT(pdlpx_mlp_set_data(N, 256, x.data(), tgt.data()) == 0, "network: set_data");
+    pdlpx_adam adam{};
+    adam.learning_rate = 1e-3;
+    adam.beta1 = 0.9;
+    adam.beta2 = 0.999;
+    adam.epsilon = 1e-8;
+    adam.final_learning_rate = 1e-4;
+    adam.weight_decay = 0.0;
+    std::vector<double> eloss(20);
+    double secs = 0.0, flops = 0.0;
+    EXPECT(pdlpx_mlp_train(N, 20, 64, &adam, 3, eloss.data(), &secs, &flops) == 0 && eloss.back() < eloss[0] &&
+               flops > 0,
+           "network: 20 epochs of Adam lower the loss");
+    std::vector<float> A(64 * 64, 0.0f), Bm(64 * 64), Cm(64 * 64);
+    for (int i = 0; i < 64; ++i) A[i * 64 + i] = 1.0f;
+    for (size_t q = 0; q < Bm.size(); ++q) Bm[q] = static_cast<float>(static_cast<int>(q % 17) - 8);
+    double ms = 0.0;
+    EXPECT(pdlpx_mlp_gemm(N, 64, 64, 64, A.data(), 0, Bm.data(), 0, Cm.data(), 2, &ms) == 0 && Cm == Bm && ms > 0,
+           "network: GEMM with the identity is exact");
+    pdlpx_mlp_destroy(N);
     std::printf("%d failure(s)\n", fails);
     return fails ? 1 : 0;
 }
