@@ -499,6 +499,16 @@ def main() -> int:
         threads = min(16, os.cpu_count() or 1)
         log(f"[bench] cpu baseline sample ({args.cpu_sample_iters or 'about 10 s of'} iterations, {threads} threads)")
         cpu = cpu_baseline(lp, tol, res.iterations, args.cpu_sample_iters, threads)
+        if cpu["kind"] == "reference" and threads > 1:
+            # the same reference on one thread (a short fixed sample); a small LP
+            # runs faster without the thread pool, so the faster one is the value
+            one = cpu_baseline(lp, tol, res.iterations, 100, 1)
+            best, other = (one, cpu) if one["value"] < cpu["value"] else (cpu, one)
+            cpu = dict(best)
+            cpu["other_thread_count"] = {k: other[k] for k in ("cores", "value", "ms_per_iteration", "setup_s",
+                                                                "sample_iterations")}
+            cpu["sample"] += (f"; also timed on {other['cores']} thread(s) ({other['value']} s); the faster of "
+                              f"{threads} threads and 1 thread is the value")
         log(f"[bench] cpu: {cpu}")
     eng.close()
 
