@@ -38,6 +38,15 @@ def instances():
     }
 
 
+def status_instances():
+    """name -> Lp whose status (infeasible / unbounded) HiGHS classifies."""
+    return {
+        "infeasible_rows": problems.infeasible_rows(),
+        "infeasible_equalities": problems.infeasible_equalities(),
+        "unbounded": problems.unbounded(),
+    }
+
+
 def highs(lp) -> dict:
     """min c x + offset s.t. rlo <= A x <= rhi, lo <= x <= hi, as linprog's
     A_ub x <= b_ub (finite upper and negated finite lower row bounds of
@@ -68,6 +77,10 @@ def main() -> None:
     for name, lp in instances().items():
         out["instances"][name] = highs(lp)
         print(name, out["instances"][name]["objective"], out["instances"][name]["message"], flush=True)
+    # linprog status: 2 = infeasible, 3 = unbounded
+    out["statuses"] = {name: highs(lp) for name, lp in status_instances().items()}
+    for name, h in out["statuses"].items():
+        print(name, h["status"], h["message"], flush=True)
     OUT.write_text(json.dumps(out, indent=1, sort_keys=True) + "\n")
 
 

@@ -19,11 +19,14 @@ import pytest
 sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
 
 import oracle_lib as ol  # noqa: E402
-from make_highs import instances  # noqa: E402
+from make_highs import instances, status_instances  # noqa: E402
 from optimisation_solver_amd import default_options  # noqa: E402
 
-HIGHS = json.loads((Path(__file__).resolve().parent / "golden" / "highs_objectives.json").read_text())["instances"]
+FIXTURE = json.loads((Path(__file__).resolve().parent / "golden" / "highs_objectives.json").read_text())
+HIGHS = FIXTURE["instances"]
 LPS = instances()
+STATUS_LPS = status_instances()
+STATUS_NAME = {2: "infeasible", 3: "unbounded"}   # linprog's status codes
 CPU_CASES = [("c0_box_2000x5000", 1e-6), ("c0_box_2000x5000", 1e-8), ("box_1500x3750_s9", 1e-6),
              ("random_feasible_300x420_s21", 1e-6), ("random_feasible_300x420_s21", 1e-8),
              ("random_feasible_260x360_s33", 1e-6), ("random_feasible_260x360_s33", 1e-8)]
@@ -68,3 +71,27 @@ def test_engine_reaches_the_highs_optimum(name, tol):
         _close(r.primal_objective, name, tol)
         m = ol.evaluate(LPS[name], _opts(tol), r.primal, r.row_dual)   # the oracle's KKT on the GPU point
         assert max(m["primal_residual"], m["dual_residual"], m["relative_gap"]) <= tol
+
+
+def _status_opts():
+    # the options of the infeasible/unbounded golden cases (tests/golden/make_golden.py)
+    return default_options(primal_tolerance=2e-6, dual_tolerance=2e-6, gap_tolerance=2e-6, thread_count=1,
+                           iteration_limit=20000, termination_check_frequency=50, polishing_iterations=2000)
+
+
+@pytest.mark.parametrize("name", sorted(FIXTURE["statuses"]))
+def test_oracle_status_agrees_with_highs(name):
+    h = FIXTURE["statuses"][name]
+    assert h["sha256"] == STATUS_LPS[name].sha256()
+    assert ol.solve(STATUS_LPS[name], _status_opts()).status_name == STATUS_NAME[h["status"]]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(FIXTURE["statuses"]))
+def test_engine_status_agrees_with_highs(name):
+    from optimisation_solver_amd import Engine
+
+    with Engine() as e:
+        e.load(STATUS_LPS[name])
+        r = e.solve(_status_opts())
+    assert r.status == STATUS_NAME[FIXTURE["statuses"][name]["status"]]
